@@ -1,0 +1,155 @@
+/*
+ * pqb_perm.h -- C ABI of the MI355X permanent engine (libpqb_perm.so).
+ *
+ * Drop-in boundary for the permanent calculators of piquassoboost.  Every
+ * entry point below names the reference interface it replaces
+ * (paths relative to the piquassoboost source tree).  Only plain C types
+ * cross this boundary: pointers, sizes and status codes.  No C++ exception
+ * leaves the library; a non-zero status comes with a message available from
+ * pqb_last_error() on the calling thread.
+ *
+ * Matrices are row-major complex128 (interleaved re, im doubles), `ld` is the
+ * row stride in complex elements (ld >= cols), exactly the layout of
+ * pic::matrix (piquassoboost/common/source/matrix_base.hpp:44-51) and of a
+ * C-contiguous numpy complex128 array.  Occupation vectors are int64, as
+ * PicState_int64 (piquassoboost/common/source/PicState.h:33).  A complex
+ * result is written as out[0] = real part, out[1] = imaginary part.
+ */
+#ifndef PQB_PERM_H
+#define PQB_PERM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PQB_OK 0
+#define PQB_ERR_ARG 1        /* invalid argument (shape, pointer, code)      */
+#define PQB_ERR_SIZE 2       /* problem outside the supported range          */
+#define PQB_ERR_STATE 3      /* occupation numbers do not add up (reference   */
+                             /* message "Number of input and output states   */
+                             /* should be equal")                             */
+#define PQB_ERR_DEVICE 4     /* no HIP device / HIP runtime error            */
+#define PQB_ERR_EXACT 5      /* exact integer mode requested on a matrix it   */
+                             /* cannot represent exactly                      */
+
+/* ---- algorithm codes: which reference calculator's edge cases apply ---- */
+/* The arithmetic is the same Gray-code sweep for all three; they differ in
+ * the shape rules of the reference:
+ *   BBFG  : BBFGPermanentCalculator.hpp:90-95 (empty -> 1, rows >= cols+2 -> 0)
+ *   GLYNN : GlynnPermanentCalculator.hpp:50-53 (same rules, recursive order)
+ *   SIMPLE: GlynnPermanentCalculatorSimple.hpp:65-70 (empty -> 1,
+ *           non-square -> 0)                                                */
+#define PQB_ALGO_BBFG 0
+#define PQB_ALGO_GLYNN 1
+#define PQB_ALGO_SIMPLE 2
+
+/* ---- precision codes ----------------------------------------------------
+ * PQB_PREC_DOUBLE   FP64 sweep, double-double reduction of all partial sums
+ *                   (reference "double": BBFGPermanentCalculator.cpp:118-121).
+ *                   Integer-valued real matrices whose permanent is provably
+ *                   representable go through the exact integer sweep instead,
+ *                   so the result is the correctly rounded exact permanent.
+ * PQB_PREC_EXTENDED Reference "long double" / MPFR codes (lib 0, 1, 8, 9).
+ *                   Same kernels as PQB_PREC_DOUBLE (the error of the FP64
+ *                   sweep with dd reduction is reported per result by the
+ *                   tests); kept as a separate code so callers keep their
+ *                   intent and a wider-accumulator kernel can slot in.
+ * PQB_PREC_FP64     Force the FP64 sweep, never the integer route.
+ * PQB_PREC_EXACT    Integer sweep modulo 2^128; fails with PQB_ERR_EXACT
+ *                   unless every entry is an integer with zero imaginary part
+ *                   and prod_i sum_j |a_ij| * 2^(rows-1) < 2^127.            */
+#define PQB_PREC_DOUBLE 0
+#define PQB_PREC_EXTENDED 1
+#define PQB_PREC_FP64 2
+#define PQB_PREC_EXACT 3
+
+/* Largest number of columns (distinct column slots for the repeated form)
+ * the kernels are instantiated for. */
+#define PQB_MAX_COLS 40
+
+/* ---- library / device --------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pqb_last_error(void);
+
+/* ABI version of this header (bumped on incompatible changes). */
+int pqb_abi_version(void);
+
+/* Number of visible HIP devices (0 when there is none; never fails). */
+int pqb_device_count(void);
+
+/* Bind the calling thread's engine to a device and create its stream and
+ * workspace.  Called implicitly by every compute entry point. */
+int pqb_init(int device);
+
+/* ---- plain permanent ------------------------------------------------------
+ * Replaces BBFGPermanentCalculator::calculate(matrix&, bool use_extended,
+ * bool use_inf)  (piquassoboost/sampling/source/BBFGPermanentCalculator.h:66,
+ * .cpp:88-126), GlynnPermanentCalculator<...>::calculate(matrix&)
+ * (GlynnPermanentCalculator.h:52, .hpp:47-57) and
+ * GlynnPermanentCalculatorSimple<...>::calculate(matrix&)
+ * (GlynnPermanentCalculatorSimple.hpp:47-53, 61-168).
+ * A: host pointer.  Synchronous.                                             */
+int pqb_perm(const double* A, int64_t rows, int64_t cols, int64_t ld, int algo,
+             int precision, double* out);
+
+/* Exact integer permanent of an integer-valued matrix (PQB_PREC_EXACT rules).
+ * words[0], words[1] = low and high 64-bit words of the two's complement
+ * 128-bit permanent; out = its correctly rounded double value.  No reference
+ * counterpart: the reference's exact results come from MPFR (use_inf,
+ * BBFGPermanentCalculator.cpp:100-107). */
+int pqb_perm_exact(const double* A, int64_t rows, int64_t cols, int64_t ld,
+                   int64_t* words, double* out);
+
+/* ---- repeated-row permanent -----------------------------------------------
+ * Replaces BBFGPermanentCalculatorRepeated::calculate(matrix&,
+ * PicState_int64& col_mult, PicState_int64& row_mult, bool use_extended,
+ * bool use_inf)  (BBFGPermanentCalculatorRepeated.h:56, .cpp:74-98) and
+ * GlynnPermanentCalculatorRepeated<...>::calculate(matrix&,
+ * PicState_int64& input_state, PicState_int64& output_state)
+ * (GlynnPermanentCalculatorRepeated.h:55-59, .hpp:49-74).
+ * Permanent of the matrix whose row i is repeated row_mult[i] times and whose
+ * column j is repeated col_mult[j] times (row_mult = output state, col_mult =
+ * input state, len(row_mult) = rows, len(col_mult) = cols).
+ * algo: PQB_ALGO_BBFG or PQB_ALGO_GLYNN (error message of that calculator). */
+int pqb_perm_repeated(const double* A, int64_t rows, int64_t cols, int64_t ld,
+                      const int64_t* row_mult, const int64_t* col_mult, int algo,
+                      int precision, double* out);
+
+/* ---- device-resident plain permanent (inputs already in HBM) -------------
+ * dA: device pointer to a packed rows x cols complex128 matrix (ld = cols).
+ * d_out: device pointer to 6 doubles: [re, im, hi_re, lo_re, hi_im, lo_im]
+ * (the double-double value of the permanent in the last four).
+ * d_work: device workspace of at least pqb_perm_device_workspace(rows, cols)
+ * bytes whose first 4 bytes are zero before the first call (each call leaves
+ * them zero again).  Asynchronous on `stream` (a hipStream_t, NULL = the
+ * null stream).  FP64 sweep, BBFG edge rules.  No reference counterpart: it
+ * is the form a caller keeping matrices on the GPU (e.g. a PyTorch tensor)
+ * uses; it allocates nothing. */
+size_t pqb_perm_device_workspace(int64_t rows, int64_t cols);
+int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
+                    void* d_work, size_t work_bytes, void* stream);
+
+/* ---- launch plan introspection (host only, no device needed) --------------
+ * Writes the sweep decomposition chosen for a rows x cols plain permanent:
+ * plan[0] = K (Gray bits per chunk), plan[1] = H (chunk-index bits),
+ * plan[2] = launched lanes, plan[3] = threads per block, plan[4] = blocks.
+ * `lanes_resident` = lanes the device keeps resident (0 = use the device). */
+int pqb_plain_plan(int64_t rows, int64_t cols, int64_t lanes_resident, int64_t* plan);
+
+/* ---- timing helper ---------------------------------------------------------
+ * Runs `iters` back-to-back pqb_perm_device launches on the engine's stream
+ * and reports the elapsed time between HIP events recorded on that stream
+ * before the first and after the last launch.  dA, d_out as above. */
+int pqb_time_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
+                         int iters, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PQB_PERM_H */

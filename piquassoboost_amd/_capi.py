@@ -1,0 +1,168 @@
+"""ctypes binding of ``libpqb_perm.so`` (the C ABI declared in include/pqb_perm.h).
+
+The library is the only compute path: when it is missing or no HIP device is
+visible, every compute call raises -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libpqb_perm.so"
+
+# status codes (include/pqb_perm.h)
+PQB_OK = 0
+PQB_ERR_ARG = 1
+PQB_ERR_SIZE = 2
+PQB_ERR_STATE = 3
+PQB_ERR_DEVICE = 4
+PQB_ERR_EXACT = 5
+
+ALGO_BBFG = 0
+ALGO_GLYNN = 1
+ALGO_SIMPLE = 2
+
+PREC_DOUBLE = 0
+PREC_EXTENDED = 1
+PREC_FP64 = 2
+PREC_EXACT = 3
+
+MAX_COLS = 40
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+# name -> (restype, argtypes); every symbol the header declares
+SIGNATURES = {
+    "pqb_last_error": (ctypes.c_char_p, []),
+    "pqb_abi_version": (ctypes.c_int, []),
+    "pqb_device_count": (ctypes.c_int, []),
+    "pqb_init": (ctypes.c_int, [ctypes.c_int]),
+    "pqb_perm": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                ctypes.c_int, _dp]),
+    "pqb_perm_exact": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _dp]),
+    "pqb_perm_repeated": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
+                                         ctypes.c_int, ctypes.c_int, _dp]),
+    "pqb_perm_device_workspace": (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
+    "pqb_perm_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "pqb_plain_plan": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p]),
+    "pqb_time_perm_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.c_int, _dp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class EngineError(Exception):
+    """Raised for every non-zero status of the C ABI (message from pqb_last_error)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def lib():
+    """Load libpqb_perm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not LIB_PATH.exists():
+                    raise RuntimeError(
+                        f"{LIB_PATH} is missing: build the HIP engine first "
+                        "(python -m piquassoboost_amd.build)"
+                    )
+                L = ctypes.CDLL(str(LIB_PATH))
+                for name, (res, args) in SIGNATURES.items():
+                    fn = getattr(L, name)
+                    fn.restype = res
+                    fn.argtypes = args
+                _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != PQB_OK:
+        msg = lib().pqb_last_error().decode(errors="replace")
+        raise EngineError(rc, msg)
+
+
+def as_matrix(A) -> np.ndarray:
+    """complex128, C-contiguous, 2-D (a 1-D array is a column, as numpy2matrix
+    does at numpy_interface.cpp:152-155)."""
+    M = np.ascontiguousarray(A, dtype=np.complex128)
+    if M.ndim == 1:
+        M = M.reshape(-1, 1)
+    if M.ndim != 2:
+        raise Exception("numpy2matrix: Wrong matrix dimension was given")
+    return M
+
+
+def as_state(s) -> np.ndarray:
+    v = np.ascontiguousarray(s, dtype=np.int64)
+    if v.ndim != 1:
+        raise Exception("numpy2PicState_int64: Wrong matrix dimension was given")
+    return v
+
+
+def device_count() -> int:
+    return int(lib().pqb_device_count())
+
+
+def permanent(A, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE) -> complex:
+    M = as_matrix(A)
+    out = (ctypes.c_double * 2)()
+    _check(lib().pqb_perm(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1], algo, precision, out))
+    return complex(out[0], out[1])
+
+
+def permanent_exact(A) -> tuple[int, float]:
+    """Exact integer permanent (two's complement 128-bit) and its rounded double."""
+    M = as_matrix(A)
+    w = (ctypes.c_int64 * 2)()
+    out = (ctypes.c_double * 2)()
+    _check(lib().pqb_perm_exact(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1], w, out))
+    v = (int(w[1]) << 64) | (int(w[0]) & 0xFFFFFFFFFFFFFFFF)
+    return v, out[0]
+
+
+def permanent_repeated(A, row_mult, col_mult, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE) -> complex:
+    M = as_matrix(A)
+    r = as_state(row_mult)
+    c = as_state(col_mult)
+    if r.shape[0] != M.shape[0] or c.shape[0] != M.shape[1]:
+        raise Exception(
+            "state lengths must match the matrix: len(output_state) == rows, len(input_state) == cols"
+        )
+    out = (ctypes.c_double * 2)()
+    _check(lib().pqb_perm_repeated(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
+                                   r.ctypes.data_as(_i64p), c.ctypes.data_as(_i64p), algo, precision, out))
+    return complex(out[0], out[1])
+
+
+def plain_plan(rows: int, cols: int, lanes_resident: int = 0) -> dict:
+    p = (ctypes.c_int64 * 5)()
+    _check(lib().pqb_plain_plan(rows, cols, lanes_resident, p))
+    return {"K": p[0], "H": p[1], "lanes": p[2], "block": p[3], "blocks": p[4]}
+
+
+def device_workspace_bytes(rows: int, cols: int) -> int:
+    return int(lib().pqb_perm_device_workspace(rows, cols))
+
+
+def permanent_device(dA_ptr: int, rows: int, cols: int, d_out_ptr: int, d_work_ptr: int, work_bytes: int,
+                     stream: int = 0) -> None:
+    """Asynchronous plain permanent on device pointers (see pqb_perm_device)."""
+    _check(lib().pqb_perm_device(dA_ptr, rows, cols, d_out_ptr, d_work_ptr, work_bytes, stream or None))
+
+
+def time_permanent_device(dA_ptr: int, rows: int, cols: int, d_out_ptr: int, iters: int) -> float:
+    ms = ctypes.c_double(0.0)
+    _check(lib().pqb_time_perm_device(dA_ptr, rows, cols, d_out_ptr, iters, ctypes.byref(ms)))
+    return ms.value

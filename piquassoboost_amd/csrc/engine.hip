@@ -1,0 +1,549 @@
+// Host engine behind the C ABI declared in include/pqb_perm.h.
+//
+// Responsibilities: argument checks and the reference's edge-case rules,
+// choice of the sweep decomposition (chunk bits K, chunk-index bits H, lane
+// count), staging host matrices into a per-process device workspace, kernel
+// launches on the engine's stream, and error reporting (status code +
+// thread-local message; no C++ exception crosses the ABI).
+//
+// The host-pointer entry points are synchronous and serialised by one mutex
+// (the reference calculators are synchronous too: the GIL is held and TBB
+// blocks, GlynnPermanentCalculator_Wrapper.hpp:200-377).  The device-pointer
+// entry point takes the caller's stream and workspace and touches no engine
+// state besides the cached launch plans.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pqb_perm.h"
+#include "pqb_device.h"
+#include "pqb_kernels.h"
+#include "pqb_engine.h"
+
+namespace pqb {
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string& msg) {
+    t_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(PQB_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Engine g_eng;
+
+// ------------------------------------------------------------------ engine --
+
+int Engine::ensure_init_locked(int dev) {
+    if (device >= 0 && (dev < 0 || dev == device)) {
+        hipError_t e = hipSetDevice(device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        return PQB_OK;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return fail(PQB_ERR_DEVICE, "no HIP device available (the permanent engine runs on the GPU only)");
+    if (dev < 0) dev = 0;
+    if (dev >= n) return fail(PQB_ERR_ARG, "device index out of range");
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+    free_buffers();
+    e = hipSetDevice(dev);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    cus = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    for (int i = 0; i <= PQB_MAX_COLS; ++i) blocks_per_cu[i] = 0;
+    device = dev;
+    return PQB_OK;
+}
+
+void Engine::free_buffers() {
+    if (dbuf) (void)hipFree(dbuf);
+    if (hbuf) (void)hipHostFree(hbuf);
+    dbuf = nullptr;
+    hbuf = nullptr;
+    dbytes = hbytes = 0;
+}
+
+int Engine::ensure_buffers_locked(size_t dneed, size_t hneed) {
+    if (dneed > dbytes) {
+        if (dbuf) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipFree(dbuf);
+            dbuf = nullptr;
+            dbytes = 0;
+        }
+        size_t sz = std::max<size_t>(dneed, (size_t)1 << 20);
+        hipError_t e = hipMalloc(&dbuf, sz);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)");
+        // the grid-reduction counter at offset 0 must start at zero
+        e = hipMemsetAsync(dbuf, 0, kCounterBytes, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMemset(workspace)");
+        dbytes = sz;
+    }
+    if (hneed > hbytes) {
+        if (hbuf) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipHostFree(hbuf);
+            hbuf = nullptr;
+            hbytes = 0;
+        }
+        size_t sz = std::max<size_t>(hneed, (size_t)1 << 20);
+        hipError_t e = hipHostMalloc(&hbuf, sz, hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)");
+        hbytes = sz;
+    }
+    return PQB_OK;
+}
+
+int64_t Engine::resident_lanes_locked(int nc) {
+    if (nc < 1 || nc > PQB_MAX_COLS) return 0;
+    if (blocks_per_cu[nc] == 0) {
+        int nb = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, plain_kernel_fn(nc), kBlock, 0);
+        if (e != hipSuccess || nb <= 0) nb = 1;
+        blocks_per_cu[nc] = nb;
+    }
+    return (int64_t)blocks_per_cu[nc] * kBlock * cus;
+}
+
+// --------------------------------------------------------------- planning --
+
+static int floor_log2(int64_t v) {
+    int r = -1;
+    while (v > 0) {
+        v >>= 1;
+        ++r;
+    }
+    return r;
+}
+
+PlainPlan make_plain_plan(int R, int NC, int64_t lanes_resident) {
+    PlainPlan pl;
+    const int T = R - 1;  // Gray bits
+    int64_t lr = std::max<int64_t>(lanes_resident, kWave);
+    const int lg = floor_log2(lr);  // lanes used: largest power of two <= resident
+    const int kmin = std::min(T, 4);
+    int K = T - lg - 2;  // about four chunks per lane when the problem is large
+    K = std::max(K, kmin);
+    K = std::min(K, std::min(T, 14));
+    pl.K = K;
+    pl.H = T - K;
+    const int64_t nchunks = (int64_t)1 << pl.H;
+    int64_t lanes = std::min<int64_t>(nchunks, (int64_t)1 << lg);
+    lanes = ((lanes + kWave - 1) / kWave) * kWave;
+    pl.block = (int)std::min<int64_t>(kBlock, lanes);
+    pl.blocks = (int)((lanes + pl.block - 1) / pl.block);
+    pl.lanes = (int64_t)pl.blocks * pl.block;
+    pl.nchunks = nchunks;
+    return pl;
+}
+
+// Column abs-sum bound: |sum_delta ...| <= 2^(R-1) prod_j sum_i |a_ij|.
+// Returns true and fills `ai` (packed R x C int32) when the matrix is
+// integer valued, real, and the exact sweep provably cannot overflow.
+static bool integer_matrix(const double* A, int64_t R, int64_t C, int64_t ld,
+                           std::vector<int>& ai, bool* is_integer) {
+    *is_integer = false;
+    ai.assign((size_t)(R * C), 0);
+    std::vector<double> colabs((size_t)C, 0.0);
+    for (int64_t i = 0; i < R; ++i) {
+        for (int64_t j = 0; j < C; ++j) {
+            const double re = A[2 * (i * ld + j)];
+            const double im = A[2 * (i * ld + j) + 1];
+            if (im != 0.0 || !(std::fabs(re) < 1073741824.0) || re != std::floor(re)) return false;
+            ai[(size_t)(i * C + j)] = (int)re;
+            colabs[(size_t)j] += std::fabs(re);
+        }
+    }
+    *is_integer = true;
+    long double lg = (long double)(R - 1);
+    for (int64_t j = 0; j < C; ++j) {
+        if (colabs[(size_t)j] >= 1073741824.0) return false;  // int32 column sums
+        if (colabs[(size_t)j] == 0.0) return true;             // zero column: perm = 0, no overflow
+        lg += std::log2((long double)colabs[(size_t)j]);
+    }
+    return lg < 126.0L;
+}
+
+typedef __int128 i128;
+
+static double i128_to_double(i128 v) { return (double)v; }
+
+// Exact route: sum_delta (prod delta) prod_j c_j  (mod 2^128) on the GPU,
+// then divide by 2^(R-1).  Caller holds the engine lock.
+static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* result) {
+    const int T = R - 1;
+    int64_t lr = g_eng.resident_lanes_locked(C);
+    const int lg = floor_log2(std::max<int64_t>(lr, kWave));
+    const int H = std::min(T, std::max(0, lg));
+    const int K = T - H;
+    if (K > 30) return fail(PQB_ERR_SIZE, "exact sweep too large");
+    const int64_t nlanes = (int64_t)1 << H;
+    const int block = (int)std::min<int64_t>(kBlock, ((nlanes + kWave - 1) / kWave) * kWave);
+    const int64_t blocks = (nlanes + block - 1) / block;
+
+    const size_t a_bytes = ai.size() * sizeof(int);
+    const size_t part_off = kCounterBytes;
+    const size_t part_bytes = (size_t)blocks * 2 * sizeof(unsigned long long);
+    const size_t out_off = part_off + ((part_bytes + 255) / 256) * 256;
+    const size_t a_off = out_off + 256;
+    int st = g_eng.ensure_buffers_locked(a_off + a_bytes, a_bytes + 256);
+    if (st) return st;
+    char* d = static_cast<char*>(g_eng.dbuf);
+    std::memcpy(g_eng.hbuf, ai.data(), a_bytes);
+    hipError_t e = hipMemcpyAsync(d + a_off, g_eng.hbuf, a_bytes, hipMemcpyHostToDevice, g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+    ExactParams p;
+    p.A = reinterpret_cast<const int*>(d + a_off);
+    p.R = R;
+    p.K = K;
+    p.H = H;
+    p.nlanes = nlanes;
+    p.partials = reinterpret_cast<unsigned long long*>(d + part_off);
+    p.counter = reinterpret_cast<unsigned*>(d);
+    p.out = reinterpret_cast<unsigned long long*>(d + out_off);
+    e = launch_plain_exact(C, p, (int)blocks, block, g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "exact sweep launch");
+    unsigned long long* hout = reinterpret_cast<unsigned long long*>(static_cast<char*>(g_eng.hbuf) + a_bytes);
+    e = hipMemcpyAsync(hout, d + out_off, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g_eng.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "exact sweep");
+    const unsigned __int128 u = ((unsigned __int128)hout[1] << 64) | hout[0];
+    const i128 sum = (i128)u;
+    const i128 mask = (((i128)1) << T) - 1;
+    if ((sum & mask) != 0) return fail(PQB_ERR_EXACT, "exact sweep: sum not divisible by 2^(rows-1)");
+    *result = sum >> T;  // arithmetic shift: exact division
+    return PQB_OK;
+}
+
+// FP64 route.  Caller holds the engine lock.
+static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* out) {
+    const int64_t lr = g_eng.resident_lanes_locked(C);
+    const PlainPlan pl = make_plain_plan(R, C, lr);
+    const size_t m2_elems = (size_t)2 * R * C;
+    const size_t m2_bytes = m2_elems * sizeof(double2);
+    const size_t part_off = kCounterBytes;
+    const size_t part_bytes = (size_t)pl.blocks * sizeof(ddc);
+    const size_t out_off = part_off + ((part_bytes + 255) / 256) * 256;
+    const size_t m2_off = out_off + 256;
+    int st = g_eng.ensure_buffers_locked(m2_off + m2_bytes, m2_bytes + 256);
+    if (st) return st;
+    double* h = static_cast<double*>(g_eng.hbuf);
+    for (int i = 0; i < R; ++i) {
+        for (int j = 0; j < C; ++j) {
+            const double re = A[2 * (i * ld + j)], im = A[2 * (i * ld + j) + 1];
+            const size_t k = (size_t)i * C + j;
+            h[2 * k] = 2.0 * re;
+            h[2 * k + 1] = 2.0 * im;
+            h[2 * (k + (size_t)R * C)] = -2.0 * re;
+            h[2 * (k + (size_t)R * C) + 1] = -2.0 * im;
+        }
+    }
+    char* d = static_cast<char*>(g_eng.dbuf);
+    hipError_t e = hipMemcpyAsync(d + m2_off, h, m2_bytes, hipMemcpyHostToDevice, g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+    PlainParams p;
+    p.M2 = reinterpret_cast<const double2*>(d + m2_off);
+    p.R = R;
+    p.K = pl.K;
+    p.H = pl.H;
+    p.nchunks = pl.nchunks;
+    p.partials = reinterpret_cast<ddc*>(d + part_off);
+    p.counter = reinterpret_cast<unsigned*>(d);
+    p.out = reinterpret_cast<double*>(d + out_off);
+    p.scale = std::ldexp(1.0, -(R - 1));
+    e = launch_plain(C, p, pl.blocks, pl.block, g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "plain sweep launch");
+    double* hout = h + 2 * m2_elems;
+    e = hipMemcpyAsync(hout, d + out_off, 6 * sizeof(double), hipMemcpyDeviceToHost, g_eng.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "plain sweep");
+    out[0] = hout[0];
+    out[1] = hout[1];
+    return PQB_OK;
+}
+
+// Shape rules of the reference calculators.  Returns 1 and fills out when
+// the result is decided without a sweep.
+static int plain_edge(int algo, int64_t rows, int64_t cols, double* out) {
+    if (algo == PQB_ALGO_SIMPLE) {
+        // GlynnPermanentCalculatorSimple.hpp:65-70
+        if (rows == 0 || cols == 0) { out[0] = 1.0; out[1] = 0.0; return 1; }
+        if (rows != cols) {
+            std::fprintf(stderr, "Matrix is not rectangular! Returning 0.\n");
+            out[0] = 0.0; out[1] = 0.0; return 1;
+        }
+        return 0;
+    }
+    // BBFGPermanentCalculator.cpp:92-95, .hpp:90-95; GlynnPermanentCalculator.hpp:50-53
+    if (rows == 0 || cols == 0) { out[0] = 1.0; out[1] = 0.0; return 1; }
+    if (rows >= cols + 2) { out[0] = 0.0; out[1] = 0.0; return 1; }
+    return 0;
+}
+
+int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int algo, int precision,
+                double* out, int64_t* words) {
+    if (!out) return fail(PQB_ERR_ARG, "output pointer is NULL");
+    if (rows < 0 || cols < 0) return fail(PQB_ERR_ARG, "negative matrix dimension");
+    if (algo < PQB_ALGO_BBFG || algo > PQB_ALGO_SIMPLE) return fail(PQB_ERR_ARG, "unknown algorithm code");
+    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_EXACT)
+        return fail(PQB_ERR_ARG, "unknown precision code");
+    if (plain_edge(algo, rows, cols, out)) {
+        if (words) {
+            const i128 v = (i128)out[0];
+            words[0] = (int64_t)(unsigned long long)v;
+            words[1] = (int64_t)(v >> 64);
+        }
+        return PQB_OK;
+    }
+    if (!A) return fail(PQB_ERR_ARG, "matrix pointer is NULL");
+    if (ld < cols) return fail(PQB_ERR_ARG, "leading dimension smaller than the number of columns");
+    if (cols > PQB_MAX_COLS)
+        return fail(PQB_ERR_SIZE, "matrix has more than " + std::to_string(PQB_MAX_COLS) + " columns");
+    const int R = (int)rows, C = (int)cols;
+
+    std::vector<int> ai;
+    bool is_int = false;
+    const bool exact_ok =
+        precision != PQB_PREC_FP64 && integer_matrix(A, rows, cols, ld, ai, &is_int);
+    if (precision == PQB_PREC_EXACT && !exact_ok)
+        return fail(PQB_ERR_EXACT, is_int ? "exact mode: the permanent may exceed the 128-bit range"
+                                          : "exact mode needs a real integer-valued matrix");
+
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    int st = g_eng.ensure_init_locked(-1);
+    if (st) return st;
+    if (exact_ok) {
+        i128 v = 0;
+        st = run_exact_locked(ai, R, C, &v);
+        if (st) return st;
+        out[0] = i128_to_double(v);
+        out[1] = 0.0;
+        if (words) {
+            words[0] = (int64_t)(unsigned long long)v;
+            words[1] = (int64_t)(v >> 64);
+        }
+        return PQB_OK;
+    }
+    return run_plain_locked(A, R, C, ld, out);
+}
+
+// Repeated-row permanent: validation and edge rules of the reference, then
+// the sweep on the matrix with rows / columns repeated by multiplicity
+// (adaptInterferometerGlynnMultiplied, BatchedPermanentCalculator.cpp:367-399).
+int repeated_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* row_mult,
+                   const int64_t* col_mult, int algo, int precision, double* out) {
+    if (!out) return fail(PQB_ERR_ARG, "output pointer is NULL");
+    if (rows < 0 || cols < 0) return fail(PQB_ERR_ARG, "negative matrix dimension");
+    if (algo != PQB_ALGO_BBFG && algo != PQB_ALGO_GLYNN) return fail(PQB_ERR_ARG, "unknown algorithm code");
+    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_EXACT) return fail(PQB_ERR_ARG, "unknown precision code");
+    if ((rows > 0 && !row_mult) || (cols > 0 && !col_mult)) return fail(PQB_ERR_ARG, "occupation pointer is NULL");
+    if (algo == PQB_ALGO_BBFG && rows == 0) {  // BBFGPermanentCalculatorRepeated.cpp:78-81
+        out[0] = 1.0; out[1] = 0.0;
+        return PQB_OK;
+    }
+    int64_t sr = 0, sc = 0;
+    for (int64_t i = 0; i < rows; ++i) {
+        if (row_mult[i] < 0) return fail(PQB_ERR_ARG, "negative occupation number");
+        sr += row_mult[i];
+    }
+    for (int64_t j = 0; j < cols; ++j) {
+        if (col_mult[j] < 0) return fail(PQB_ERR_ARG, "negative occupation number");
+        sc += col_mult[j];
+    }
+    if (sr != sc)
+        return fail(PQB_ERR_STATE, algo == PQB_ALGO_BBFG
+            ? "BBFGPermanentCalculatorRepeated_Tasks::calculate:  Number of input and output states should be equal"
+            : "GlynnPermanentCalculatorRepeated::calculate:  Number of input and output states should be equal");
+    if (rows == 0 || cols == 0 || sr == 0) {  // BBFGPermanentCalculatorRepeated.hpp:126-128, GlynnPermanentCalculatorRepeated.hpp:62-63
+        out[0] = 1.0; out[1] = 0.0;
+        return PQB_OK;
+    }
+    if (!A) return fail(PQB_ERR_ARG, "matrix pointer is NULL");
+    if (ld < cols) return fail(PQB_ERR_ARG, "leading dimension smaller than the number of columns");
+    if (sr > PQB_MAX_COLS)
+        return fail(PQB_ERR_SIZE, "more than " + std::to_string(PQB_MAX_COLS) + " photons");
+    const int64_t N = sr;
+    std::vector<double> E((size_t)(2 * N * N));
+    int64_t r = 0;
+    for (int64_t i = 0; i < rows; ++i)
+        for (int64_t a = 0; a < row_mult[i]; ++a, ++r) {
+            int64_t c = 0;
+            for (int64_t j = 0; j < cols; ++j)
+                for (int64_t b = 0; b < col_mult[j]; ++b, ++c) {
+                    E[(size_t)(2 * (r * N + c))] = A[2 * (i * ld + j)];
+                    E[(size_t)(2 * (r * N + c) + 1)] = A[2 * (i * ld + j) + 1];
+                }
+        }
+    return plain_entry(E.data(), N, N, N, PQB_ALGO_BBFG, precision, out, nullptr);
+}
+
+}  // namespace pqb
+
+using namespace pqb;
+
+extern "C" {
+
+const char* pqb_last_error(void) { return t_err.c_str(); }
+
+int pqb_abi_version(void) { return 1; }
+
+int pqb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pqb_init(int device) {
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    return g_eng.ensure_init_locked(device);
+}
+
+int pqb_perm(const double* A, int64_t rows, int64_t cols, int64_t ld, int algo, int precision,
+             double* out) {
+    try {
+        return plain_entry(A, rows, cols, ld, algo, precision, out, nullptr);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+int pqb_perm_exact(const double* A, int64_t rows, int64_t cols, int64_t ld, int64_t* words,
+                   double* out) {
+    if (!words) return fail(PQB_ERR_ARG, "words pointer is NULL");
+    try {
+        return plain_entry(A, rows, cols, ld, PQB_ALGO_BBFG, PQB_PREC_EXACT, out, words);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+int pqb_perm_repeated(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* row_mult,
+                      const int64_t* col_mult, int algo, int precision, double* out) {
+    try {
+        return repeated_entry(A, rows, cols, ld, row_mult, col_mult, algo, precision, out);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+size_t pqb_perm_device_workspace(int64_t rows, int64_t cols) {
+    if (rows < 1 || cols < 1) return kCounterBytes;
+    return kCounterBytes + (size_t)kMaxBlocks * sizeof(ddc) + (size_t)2 * rows * cols * sizeof(double2);
+}
+
+int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out, void* d_work,
+                    size_t work_bytes, void* stream) {
+    if (!dA || !d_out || !d_work) return fail(PQB_ERR_ARG, "NULL device pointer");
+    if (rows < 1 || cols < 1 || rows >= cols + 2)
+        return fail(PQB_ERR_ARG, "pqb_perm_device needs 1 <= rows <= cols + 1");
+    if (cols > PQB_MAX_COLS)
+        return fail(PQB_ERR_SIZE, "matrix has more than " + std::to_string(PQB_MAX_COLS) + " columns");
+    if (work_bytes < pqb_perm_device_workspace(rows, cols)) return fail(PQB_ERR_ARG, "workspace too small");
+    const int R = (int)rows, C = (int)cols;
+    int64_t lr;
+    {
+        std::lock_guard<std::mutex> lk(g_eng.mu);
+        int st = g_eng.ensure_init_locked(-1);
+        if (st) return st;
+        lr = g_eng.resident_lanes_locked(C);
+    }
+    const PlainPlan pl = make_plain_plan(R, C, lr);
+    if (pl.blocks > kMaxBlocks) return fail(PQB_ERR_SIZE, "launch plan exceeds the workspace layout");
+    char* w = static_cast<char*>(d_work);
+    double2* m2 = reinterpret_cast<double2*>(w + kCounterBytes + (size_t)kMaxBlocks * sizeof(ddc));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = launch_plain_prep(reinterpret_cast<const double2*>(dA), m2, R * C, st);
+    if (e != hipSuccess) return hip_fail(e, "prep launch");
+    PlainParams p;
+    p.M2 = m2;
+    p.R = R;
+    p.K = pl.K;
+    p.H = pl.H;
+    p.nchunks = pl.nchunks;
+    p.partials = reinterpret_cast<ddc*>(w + kCounterBytes);
+    p.counter = reinterpret_cast<unsigned*>(w);
+    p.out = d_out;
+    p.scale = std::ldexp(1.0, -(R - 1));
+    e = launch_plain(C, p, pl.blocks, pl.block, st);
+    if (e != hipSuccess) return hip_fail(e, "plain sweep launch");
+    return PQB_OK;
+}
+
+int pqb_plain_plan(int64_t rows, int64_t cols, int64_t lanes_resident, int64_t* plan) {
+    if (!plan) return fail(PQB_ERR_ARG, "plan pointer is NULL");
+    if (rows < 1 || cols < 1 || rows >= cols + 2 || cols > PQB_MAX_COLS)
+        return fail(PQB_ERR_ARG, "no sweep for this shape");
+    int64_t lr = lanes_resident;
+    if (lr <= 0) {
+        std::lock_guard<std::mutex> lk(g_eng.mu);
+        int st = g_eng.ensure_init_locked(-1);
+        if (st) return st;
+        lr = g_eng.resident_lanes_locked((int)cols);
+    }
+    const PlainPlan pl = make_plain_plan((int)rows, (int)cols, lr);
+    plan[0] = pl.K;
+    plan[1] = pl.H;
+    plan[2] = pl.lanes;
+    plan[3] = pl.block;
+    plan[4] = pl.blocks;
+    return PQB_OK;
+}
+
+int pqb_time_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out, int iters,
+                         double* ms) {
+    if (!ms || iters < 1) return fail(PQB_ERR_ARG, "bad timing arguments");
+    const size_t wb = pqb_perm_device_workspace(rows, cols);
+    void* w = nullptr;
+    hipStream_t st;
+    {
+        std::lock_guard<std::mutex> lk(g_eng.mu);
+        int s = g_eng.ensure_init_locked(-1);
+        if (s) return s;
+        st = g_eng.stream;
+    }
+    hipError_t e = hipMalloc(&w, wb);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(timing workspace)");
+    e = hipMemsetAsync(w, 0, kCounterBytes, st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int rc = PQB_OK;
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int i = 0; e == hipSuccess && rc == PQB_OK && i < iters; ++i)
+        rc = pqb_perm_device(dA, rows, cols, d_out, w, wb, st);
+    if (e == hipSuccess && rc == PQB_OK) e = hipEventRecord(e1, st);
+    if (e == hipSuccess && rc == PQB_OK) e = hipEventSynchronize(e1);
+    float f = 0.f;
+    if (e == hipSuccess && rc == PQB_OK) e = hipEventElapsedTime(&f, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(w);
+    if (rc != PQB_OK) return rc;
+    if (e != hipSuccess) return hip_fail(e, "timing");
+    *ms = f;
+    return PQB_OK;
+}
+
+}  // extern "C"

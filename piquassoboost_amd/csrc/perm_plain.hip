@@ -1,0 +1,368 @@
+// Plain Glynn / BBFG permanent: the binary reflected Gray-code sweep over the
+// 2^(R-1) sign vectors delta in {+1,-1}^R with delta_0 = +1,
+//
+//     perm(A) = 2^-(R-1) * sum_delta (prod_i delta_i) prod_j sum_i delta_i a_ij,
+//
+// restated from BBFGPermanentCalculator.hpp:87-235 (sweep :179-221, seed
+// :143-176, edge cases :90-95).  As in the reference, Gray bit b is tied to
+// row R-1-b (:150-167, :186-191), a flip of row i changes every column sum by
+// -/+ 2 a_ij (:205-216) and the running product is taken over all columns.
+//
+// MI355X layout of the sweep (one permanent per launch):
+//
+// * The index space is split into 2^H chunks of 2^K consecutive Gray steps.
+//   A chunk is identified by its H high bits (rows 1 .. R-1-K); the K low bits
+//   (rows R-K .. R-1) are walked inside the chunk, always starting from
+//   delta = +1.  Every lane therefore walks the SAME K-bit reflected Gray
+//   sequence, so the flipped row and its direction are wave-uniform: the row
+//   of +/-2A is fetched with scalar loads and used as an SGPR operand, and the
+//   Gray bookkeeping (ctz, parity) runs on the SALU.
+// * A lane seeds its column sums directly from its chunk's high bits (like the
+//   reference's per-chunk seed, BBFGPermanentCalculator.hpp:143-167) and takes
+//   chunks g, g+L, g+2L, ... (L = launched lanes).  The seed costs 2*R*NC FMAs,
+//   about 1% of a 2^10-step chunk at NC=30.
+// * The product of the NC column sums is formed in complex FP64 as four
+//   independent chains (for ILP) and the last multiply is fused into the
+//   accumulation (4 FMAs), giving 6*NC-4 VALU instructions per Gray step
+//   against 8*NC-4 algorithmic flops.
+// * Terms of even and odd Gray index go to two accumulators (no sign work).
+//   Chunk totals are combined per lane in FP64; everything above the lane
+//   (wave, block, grid) is summed in double-double, and the last block to
+//   finish reduces the per-block partials and writes the result.
+//
+// A second kernel evaluates the same sum exactly for integer matrices, with
+// integer column sums and products/sums modulo 2^128 (two's complement
+// wrap-around); the true sum 2^(R-1) perm is recovered exactly whenever
+// |2^(R-1) perm| < 2^127, which the host checks with the row-sum bound.
+
+#include "pqb_device.h"
+#include "pqb_kernels.h"
+
+namespace pqb {
+
+// Product of cr[0..M) (complex), as up to four independent chains.
+template <int M, int NC>
+__device__ __forceinline__ void prod_first(const double (&cr)[NC], const double (&ci)[NC],
+                                           double& pr, double& pi) {
+    static_assert(M >= 1 && M <= NC, "bad product length");
+    constexpr int NCH = (M >= 12) ? 4 : ((M >= 4) ? 2 : 1);
+    double qr[NCH], qi[NCH];
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+        const int lo = q * M / NCH;
+        const int hi = (q + 1) * M / NCH;
+        qr[q] = cr[lo];
+        qi[q] = ci[lo];
+#pragma unroll
+        for (int j = lo + 1; j < hi; ++j) cmul(qr[q], qi[q], cr[j], ci[j]);
+    }
+    if constexpr (NCH == 4) {
+        cmul(qr[0], qi[0], qr[1], qi[1]);
+        cmul(qr[2], qi[2], qr[3], qi[3]);
+        cmul(qr[0], qi[0], qr[2], qi[2]);
+    } else if constexpr (NCH == 2) {
+        cmul(qr[0], qi[0], qr[1], qi[1]);
+    }
+    pr = qr[0];
+    pi = qi[0];
+}
+
+// acc += prod_j c_j  (last factor fused into the accumulation).
+template <int NC>
+__device__ __forceinline__ void add_term(const double (&cr)[NC], const double (&ci)[NC],
+                                         double& ar, double& ai) {
+    if constexpr (NC == 1) {
+        ar += cr[0];
+        ai += ci[0];
+    } else {
+        double pr, pi;
+        prod_first<NC - 1, NC>(cr, ci, pr, pi);
+        const double lr = cr[NC - 1], li = ci[NC - 1];
+        ar = fma(pr, lr, ar);
+        ar = fma(-pi, li, ar);
+        ai = fma(pr, li, ai);
+        ai = fma(pi, lr, ai);
+    }
+}
+
+// colsum += row (row is wave-uniform: scalar loads + SGPR operands).
+template <int NC>
+__device__ __forceinline__ void add_row(double (&cr)[NC], double (&ci)[NC],
+                                        const double2* __restrict__ row) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const double2 v = row[j];
+        cr[j] += v.x;
+        ci[j] += v.y;
+        // keep at most 16 columns (64 SGPRs) of the row in flight
+        if ((j & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// colsum += s * row, s = +/-1 per lane.
+template <int NC>
+__device__ __forceinline__ void add_row_signed(double (&cr)[NC], double (&ci)[NC],
+                                               const double2* __restrict__ row, double s) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const double2 v = row[j];
+        cr[j] = fma(s, v.x, cr[j]);
+        ci[j] = fma(s, v.y, ci[j]);
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void plain_sweep_kernel(PlainParams p) {
+    __shared__ ddc red[256 / kWave];
+
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t L = (int64_t)gridDim.x * blockDim.x;
+
+    const int R = p.R;
+    const int K = p.K;
+    const int H = p.H;
+    const unsigned rowb = NC * sizeof(double2);
+    // byte offsets of the +2A / -2A rows: row r of +2A at r*rowb, of -2A at (R+r)*rowb
+    const char* __restrict__ Mb = reinterpret_cast<const char*>(p.M2);
+    const unsigned neg = (unsigned)R * rowb;
+    const unsigned low_p = (unsigned)(R - 1) * rowb;  // +2A[R-1]
+    const unsigned low_m = neg + low_p;               // -2A[R-1]
+    const unsigned W = 1u << K;
+
+    double tr = 0.0, ti = 0.0;  // lane total over its chunks
+    for (int64_t c = g; c < p.nchunks; c += L) {
+        // ---- seed: delta_0 = +1, low rows +1, high row R-1-K-h is -1 iff bit h of c.
+        // Built from the +2A rows and halved at the end (scaling by 2 is exact,
+        // so this equals seeding from A as BBFGPermanentCalculator.hpp:143-167).
+        double cr[NC], ci[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            const double2 v = reinterpret_cast<const double2*>(Mb)[j];
+            cr[j] = v.x;
+            ci[j] = v.y;
+        }
+        for (int i = R - K; i < R; ++i)
+            add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + (size_t)i * rowb));
+        for (int h = 0; h < H; ++h) {
+            const int row = R - 1 - K - h;
+            const double s = ((c >> h) & 1) ? -1.0 : 1.0;
+            add_row_signed<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + (size_t)row * rowb), s);
+        }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            cr[j] *= 0.5;
+            ci[j] *= 0.5;
+        }
+
+        double pr = 0.0, pi = 0.0, mr = 0.0, mi = 0.0;
+        // t = 0: the seed itself (even Gray index)
+        add_term<NC>(cr, ci, pr, pi);
+        if (K >= 1) {
+            // t = 1: bit 0 (row R-1) goes to delta = -1
+            add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + low_m));
+            add_term<NC>(cr, ci, mr, mi);
+            for (unsigned t = 2; t < W; t += 2) {
+                const unsigned b = __builtin_ctz(t);
+                const unsigned nb = ((t >> b) ^ (t >> (b + 1))) & 1u;
+                const unsigned off = (nb ? neg : 0u) + (unsigned)(R - 1 - (int)b) * rowb;
+                add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off));
+                add_term<NC>(cr, ci, pr, pi);
+                const unsigned off0 = ((t >> 1) & 1u) ? low_p : low_m;
+                add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off0));
+                add_term<NC>(cr, ci, mr, mi);
+            }
+        }
+        // sign of the chunk's first term: parity of its high bits
+        const double cs = (__popcll((unsigned long long)c) & 1) ? -1.0 : 1.0;
+        tr = fma(cs, pr - mr, tr);
+        ti = fma(cs, pi - mi, ti);
+    }
+
+    ddc v = block_reduce(ddc_from(tr, ti), red);
+
+    // ---- grid reduction: last block sums the per-block partials ----
+    __shared__ bool is_last;
+    if (threadIdx.x == 0) {
+        p.partials[blockIdx.x] = v;
+        __threadfence();
+        const unsigned done = atomicAdd(p.counter, 1u);
+        is_last = (done == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        ddc s = {0.0, 0.0, 0.0, 0.0};
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            const ddc q = p.partials[b];
+            ddc_add(s, q);
+        }
+        s = block_reduce(s, red);
+        if (threadIdx.x == 0) {
+            s = ddc_scale(s, p.scale);
+            p.out[0] = s.hr + s.lr;
+            p.out[1] = s.hi + s.li;
+            p.out[2] = s.hr;
+            p.out[3] = s.lr;
+            p.out[4] = s.hi;
+            p.out[5] = s.li;
+            *p.counter = 0u;  // leave the counter ready for the next launch
+        }
+    }
+}
+
+// M2 = [+2A ; -2A] from a packed R x NC device matrix.
+__global__ void plain_prep_kernel(const double2* __restrict__ A, double2* __restrict__ M2, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double2 v = A[i];
+        M2[i] = make_double2(2.0 * v.x, 2.0 * v.y);
+        M2[n + i] = make_double2(-2.0 * v.x, -2.0 * v.y);
+    }
+}
+
+hipError_t launch_plain_prep(const double2* A, double2* M2, int n, hipStream_t st) {
+    const int bd = 256;
+    hipLaunchKernelGGL(plain_prep_kernel, dim3((n + bd - 1) / bd), dim3(bd), 0, st, A, M2, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- exact int --
+
+typedef unsigned __int128 u128;
+
+template <int NC>
+__global__ __launch_bounds__(256) void plain_exact_kernel(ExactParams p) {
+    __shared__ u128 red[256 / kWave];
+
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = g < p.nlanes;
+    const int64_t gl = active ? g : 0;
+    const int R = p.R, K = p.K, H = p.H;
+    const int* __restrict__ A = p.A;  // R x NC integers
+
+    int c[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) c[j] = A[j];
+    for (int i = R - K; i < R; ++i) {
+#pragma unroll
+        for (int j = 0; j < NC; ++j) c[j] += A[i * NC + j];
+    }
+    for (int h = 0; h < H; ++h) {
+        const int row = R - 1 - K - h;
+        const int s = ((gl >> h) & 1) ? -1 : 1;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) c[j] += s * A[row * NC + j];
+    }
+    const bool lane_neg = __popcll((unsigned long long)gl) & 1;
+
+    u128 acc = 0;
+    const int64_t W = (int64_t)1 << K;
+    for (int64_t t = 0; t < W; ++t) {
+        if (t > 0) {
+            const int b = __builtin_ctzll((unsigned long long)t);
+            const int nb = (int)(((t >> b) ^ (t >> (b + 1))) & 1);
+            const int row = R - 1 - b;
+            const int s = nb ? -2 : 2;
+#pragma unroll
+            for (int j = 0; j < NC; ++j) c[j] += s * A[row * NC + j];
+        }
+        u128 pr = (u128)(__int128)c[0];
+#pragma unroll
+        for (int j = 1; j < NC; ++j) pr *= (u128)(__int128)c[j];
+        if (t & 1) acc -= pr; else acc += pr;
+    }
+    if (lane_neg) acc = (u128)0 - acc;
+    if (!active) acc = 0;
+
+    // block sum (wrap-around arithmetic is associative: order does not matter)
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    unsigned long long lo = (unsigned long long)acc, hi = (unsigned long long)(acc >> 64);
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+        const unsigned long long olo = __shfl_xor(lo, m, kWave);
+        const unsigned long long ohi = __shfl_xor(hi, m, kWave);
+        const u128 s = (((u128)hi << 64) | lo) + (((u128)ohi << 64) | olo);
+        lo = (unsigned long long)s;
+        hi = (unsigned long long)(s >> 64);
+    }
+    if (lane == 0) red[wid] = ((u128)hi << 64) | lo;
+    __syncthreads();
+    __shared__ bool is_last;
+    if (threadIdx.x == 0) {
+        u128 s = 0;
+        const int nw = (blockDim.x + kWave - 1) / kWave;
+        for (int w = 0; w < nw; ++w) s += red[w];
+        p.partials[2 * blockIdx.x] = (unsigned long long)s;
+        p.partials[2 * blockIdx.x + 1] = (unsigned long long)(s >> 64);
+        __threadfence();
+        const unsigned done = atomicAdd(p.counter, 1u);
+        is_last = (done == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last && threadIdx.x == 0) {
+        __threadfence();
+        u128 s = 0;
+        for (unsigned b = 0; b < gridDim.x; ++b)
+            s += ((u128)p.partials[2 * b + 1] << 64) | p.partials[2 * b];
+        p.out[0] = (unsigned long long)s;
+        p.out[1] = (unsigned long long)(s >> 64);
+        *p.counter = 0u;
+    }
+}
+
+// ------------------------------------------------------------ dispatch --
+
+template <int NC>
+static hipError_t launch_plain_nc(const PlainParams& p, int nblocks, int bdim, hipStream_t st) {
+    hipLaunchKernelGGL(plain_sweep_kernel<NC>, dim3(nblocks), dim3(bdim), 0, st, p);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launch_exact_nc(const ExactParams& p, int nblocks, int bdim, hipStream_t st) {
+    hipLaunchKernelGGL(plain_exact_kernel<NC>, dim3(nblocks), dim3(bdim), 0, st, p);
+    return hipGetLastError();
+}
+
+template <int NC>
+static const void* plain_fn_nc() {
+    return reinterpret_cast<const void*>(&plain_sweep_kernel<NC>);
+}
+
+#define PQB_NC_CASES(X) \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+    X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30)    \
+    X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39) X(40)
+
+hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, hipStream_t st) {
+    switch (nc) {
+#define PQB_CASE(N) \
+    case N: return launch_plain_nc<N>(p, nblocks, bdim, st);
+        PQB_NC_CASES(PQB_CASE)
+#undef PQB_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+const void* plain_kernel_fn(int nc) {
+    switch (nc) {
+#define PQB_CASE(N) \
+    case N: return plain_fn_nc<N>();
+        PQB_NC_CASES(PQB_CASE)
+#undef PQB_CASE
+        default: return nullptr;
+    }
+}
+
+hipError_t launch_plain_exact(int nc, const ExactParams& p, int nblocks, int bdim, hipStream_t st) {
+    switch (nc) {
+#define PQB_CASE(N) \
+    case N: return launch_exact_nc<N>(p, nblocks, bdim, st);
+        PQB_NC_CASES(PQB_CASE)
+#undef PQB_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace pqb

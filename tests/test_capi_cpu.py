@@ -1,0 +1,134 @@
+"""C-ABI boundary checks that need no GPU: the library loads, exports every
+symbol include/pqb_perm.h declares, and applies the reference's argument and
+edge-case rules before any device work (these paths never launch a kernel)."""
+
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from piquassoboost_amd import _capi as C
+from piquassoboost_amd.sampling import Boson_Sampling_Utilities as BSU
+from piquassoboost_amd.sampling.permanent_calculators import permanent_CPU_repeated_double
+
+HEADER = ROOT / "include" / "pqb_perm.h"
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pqb_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    L = C.lib()
+    names = declared_symbols()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(L, n), n
+    # and the binding covers every declaration
+    assert set(names) <= set(C.SIGNATURES), set(names) - set(C.SIGNATURES)
+
+
+def test_abi_version_and_device_query():
+    assert C.lib().pqb_abi_version() == 1
+    assert C.device_count() >= 0  # never fails, 0 on a machine without a GPU
+
+
+def test_error_text_empty_before_failure():
+    assert isinstance(C.lib().pqb_last_error(), bytes)
+
+
+def _raw_perm(A, algo=C.ALGO_BBFG, prec=C.PREC_DOUBLE):
+    M = np.ascontiguousarray(A, dtype=np.complex128).reshape(A.shape[0], -1) if A.size else np.zeros((A.shape[0], A.shape[1]), np.complex128)
+    out = (ctypes.c_double * 2)()
+    rc = C.lib().pqb_perm(M.ctypes.data_as(C._dp), M.shape[0], M.shape[1], max(M.shape[1], 0), algo, prec, out)
+    return rc, complex(out[0], out[1])
+
+
+def test_edge_rules_without_device():
+    # BBFGPermanentCalculator.hpp:90-95, GlynnPermanentCalculator.hpp:50-53
+    assert _raw_perm(np.zeros((0, 0))) == (0, 1.0)
+    assert _raw_perm(np.zeros((0, 3))) == (0, 1.0)
+    assert _raw_perm(np.ones((5, 3))) == (0, 0.0)
+    assert _raw_perm(np.ones((5, 3)), C.ALGO_GLYNN) == (0, 0.0)
+    # GlynnPermanentCalculatorSimple.hpp:65-70: non-square -> 0
+    assert _raw_perm(np.ones((2, 3)), C.ALGO_SIMPLE) == (0, 0.0)
+
+
+def test_argument_errors():
+    out = (ctypes.c_double * 2)()
+    A = np.ones((2, 2), dtype=np.complex128)
+    L = C.lib()
+    assert L.pqb_perm(A.ctypes.data_as(C._dp), -1, 2, 2, 0, 0, out) == C.PQB_ERR_ARG
+    assert L.pqb_perm(A.ctypes.data_as(C._dp), 2, 2, 1, 0, 0, out) == C.PQB_ERR_ARG  # ld < cols
+    assert L.pqb_perm(A.ctypes.data_as(C._dp), 2, 2, 2, 9, 0, out) == C.PQB_ERR_ARG  # algo
+    assert L.pqb_perm(A.ctypes.data_as(C._dp), 2, 2, 2, 0, 9, out) == C.PQB_ERR_ARG  # precision
+    assert L.pqb_perm(None, 2, 2, 2, 0, 0, out) == C.PQB_ERR_ARG
+    big = np.ones((41, 41), dtype=np.complex128)
+    assert L.pqb_perm(big.ctypes.data_as(C._dp), 41, 41, 41, 0, 0, out) == C.PQB_ERR_SIZE
+    assert "40 columns" in L.pqb_last_error().decode()
+
+
+def test_exact_mode_rejects_non_integer():
+    with pytest.raises(C.EngineError) as ei:
+        C.permanent(np.array([[0.5, 1], [1, 1]]), C.ALGO_BBFG, C.PREC_EXACT)
+    assert ei.value.code == C.PQB_ERR_EXACT
+
+
+def test_repeated_state_errors_match_reference():
+    # BBFGPermanentCalculatorRepeated.hpp:119-124
+    with pytest.raises(Exception, match=r"^BBFGPermanentCalculatorRepeated_Tasks::calculate:  Number of input and "
+                                        r"output states should be equal$"):
+        BSU.BBFGRepeatedPermanentCalculatorDouble(np.eye(2), np.array([1, 1]), np.array([2, 1])).calculate()
+    # GlynnPermanentCalculatorRepeated.hpp:55-60
+    with pytest.raises(Exception, match=r"^GlynnPermanentCalculatorRepeated::calculate:  Number of input and "
+                                        r"output states should be equal$"):
+        BSU.GlynnRepeatedPermanentCalculatorDouble(np.eye(2), np.array([1, 1]), np.array([2, 1])).calculate()
+    with pytest.raises(Exception, match="Number of input and output states should be equal"):
+        permanent_CPU_repeated_double(np.eye(2), np.array([1, 1]), np.array([0, 1]))
+
+
+def test_repeated_edge_rules_without_device():
+    # zero photons -> 1 (unittests/test_permanent.py:55-78)
+    v = BSU.BBFGRepeatedPermanentCalculatorDouble(np.eye(3), np.zeros(3, np.int64), np.zeros(3, np.int64)).calculate()
+    assert v == 1.0
+    v = BSU.GlynnRepeatedPermanentCalculator(np.eye(3), np.zeros(3, np.int64), np.zeros(3, np.int64)).calculate()
+    assert v == 1.0
+
+
+def test_wrong_lib_codes():
+    for lib in (2, 3, 4, 5, 9, 12, -1):
+        with pytest.raises(Exception, match="Wrong value set for permanent library."):
+            BSU.GlynnPermanentCalculator_wrapper(np.eye(2), lib)
+    for lib in (2, 3, 4, 5, 10, 11, 12):
+        with pytest.raises(Exception, match="Wrong value set for permanent library."):
+            BSU.ChinHuhPermanentCalculator_wrapper(lib, np.eye(2), np.array([1, 1]), np.array([1, 1]))
+    for cls in (BSU.GlynnPermanentSingleDFE, BSU.GlynnPermanentDualDFE):
+        with pytest.raises(Exception, match="Wrong value set for permanent library."):
+            cls(np.eye(2))
+
+
+def test_state_type_checks():
+    with pytest.raises(Exception, match="The input state should be a numpy array"):
+        BSU.BBFGRepeatedPermanentCalculatorDouble(np.eye(2), [1, 1], np.array([1, 1]))
+
+
+def test_state_length_mismatch_raises():
+    with pytest.raises(Exception, match="state lengths must match"):
+        BSU.BBFGRepeatedPermanentCalculatorDouble(np.eye(3), np.array([1, 1]), np.array([1, 1])).calculate()
+
+
+def test_plan_is_host_only():
+    p = C.plain_plan(30, 30, lanes_resident=131072)
+    assert p["K"] + p["H"] == 29
+    assert p["lanes"] <= 131072 and p["lanes"] % 64 == 0
+    assert p["blocks"] * p["block"] == p["lanes"]
+    small = C.plain_plan(3, 3, lanes_resident=131072)
+    assert small["K"] + small["H"] == 2 and small["lanes"] == 64
+    for n in range(1, 41):
+        q = C.plain_plan(n, n, lanes_resident=196608)
+        assert q["K"] + q["H"] == n - 1
+        assert 0 <= q["K"] <= 14

@@ -1,0 +1,181 @@
+"""GPU parity of the plain permanent (FP64 sweep and exact integer sweep)
+against the CPU oracle and the golden fixtures.  Every call goes through the
+C ABI of libpqb_perm.so."""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, random_complex, rel_err
+from piquassoboost_amd import _capi as C
+from piquassoboost_amd.sampling import Boson_Sampling_Utilities as BSU
+
+pytestmark = pytest.mark.gpu
+
+# The FP64 sweep is compared with the double-double oracle.  Tolerances: the
+# north star's 1e-12 (C1, n=20) and 1e-10 (C2a, n=28/30) relative errors; the
+# small random cases use 1e-12 as well.
+TOL_SMALL = 1e-12
+
+
+def test_device_visible():
+    assert C.device_count() >= 1
+
+
+@pytest.mark.parametrize("n", list(range(1, 17)) + [18, 21, 24])
+def test_square_random_vs_oracle(n):
+    rng = np.random.default_rng(1000 + n)
+    A = random_complex(rng, n, n)
+    ref = oracle.bbfg(A, "dd")
+    got = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
+    print(f"n={n} rel_err={rel_err(got, ref):.3e}", flush=True)
+    assert rel_err(got, ref) < TOL_SMALL
+
+
+@pytest.mark.parametrize("shape", [(2, 1), (3, 2), (5, 4), (9, 8), (2, 5), (4, 9), (7, 12), (1, 6), (13, 14)])
+def test_rectangular_vs_oracle(shape):
+    """rows = cols+1 and rows < cols follow the same sum (BBFGPermanentCalculator.hpp:87-235)."""
+    r, c = shape
+    rng = np.random.default_rng(r * 100 + c)
+    A = random_complex(rng, r, c)
+    ref = oracle.bbfg(A, "dd")
+    got = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
+    # rows = cols + 1 sums to zero up to rounding: compare on the scale of the terms
+    scale = max(abs(ref), np.prod(np.abs(A).sum(axis=0)) * 1e-3)
+    assert abs(got - ref) <= TOL_SMALL * scale, (got, ref)
+
+
+def test_strided_input_matches_contiguous():
+    rng = np.random.default_rng(3)
+    big = random_complex(rng, 12, 20)
+    A = big[:, 3:15]  # non-contiguous view, converted by the binding
+    assert C.permanent(A) == C.permanent(np.ascontiguousarray(A))
+
+
+def test_kat_plain_all_libs(kat_cases):
+    c = [k for k in kat_cases if k["name"] == "no_repetition_6x6"][0]
+    for cls in (BSU.GlynnPermanent, BSU.GlynnPermanentInf, BSU.GlynnPermanentDouble, BSU.BBFGPermanentDouble,
+                BSU.BBFGPermanentLongDouble, BSU.GlynnPermanentSimpleDouble, BSU.GlynnPermanentSimpleLongDouble):
+        v = cls(c["matrix"]).calculate()
+        assert isinstance(v, complex)
+        assert rel_err(v, c["value"]) < 1e-12, cls.__name__
+
+
+def test_list_batched_plain():
+    rng = np.random.default_rng(8)
+    mats = [random_complex(rng, n, n) for n in (3, 5, 8, 2)]
+    res = BSU.BBFGPermanentDouble(mats).calculate()
+    assert isinstance(res, list) and len(res) == 4
+    for m, v in zip(mats, res):
+        assert rel_err(v, oracle.bbfg(m, "dd")) < TOL_SMALL
+
+
+def test_c1_haar_n20(haar_blocks):
+    """Config 1: n=20 block of a 40-mode Haar unitary, 1e-12 vs the reference
+    GlynnPermanentCalculator (long double) restated."""
+    A = haar_blocks["c1_n20"]
+    ref = complex(*haar_blocks["c1_n20_glynn_ld"])
+    for lib_cls in (BSU.GlynnPermanent, BSU.BBFGPermanentDouble):
+        got = lib_cls(A).calculate()
+        err = rel_err(got, ref)
+        print(f"C1 {lib_cls.__name__}: rel_err={err:.3e}", flush=True)
+        assert err < 1e-12
+
+
+@pytest.mark.parametrize("name", ["c2a_n28", "c2a_n30"])
+def test_c2a_haar_large(haar_blocks, name):
+    """Config 2a: n=28 / n=30 blocks of a 60-mode Haar unitary within 1e-10 of
+    the double-double oracle."""
+    A = haar_blocks[name]
+    h = haar_blocks[name + "_dd"]
+    ref = complex(h[0] + h[1], h[2] + h[3])
+    got = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
+    err = rel_err(got, ref)
+    print(f"{name}: got={got!r} ref={ref!r} rel_err={err:.3e}", flush=True)
+    assert err < 1e-10
+
+
+def test_exact_small_integer_matrices():
+    rng = np.random.default_rng(21)
+    for n in range(1, 15):
+        A = rng.integers(-2, 3, size=(n, n))
+        exact = oracle.int_permanent(A)
+        v, d = C.permanent_exact(A.astype(float))
+        assert v == exact, n
+        assert d == float(exact)
+
+
+def test_fp64_path_bit_exact_on_01_below_2p53():
+    """0/1 matrices whose permanents fit in 2^53: the double result equals the
+    integer permanent bit for bit (default precision routes integer input to
+    the exact sweep)."""
+    rng = np.random.default_rng(22)
+    for n in (8, 12, 16, 20, 22):
+        A = (rng.random((n, n)) < 0.35).astype(float)
+        exact = oracle.int_permanent(A.astype(np.int64))
+        assert exact < 2 ** 53
+        got = BSU.BBFGPermanentDouble(A).calculate()
+        assert got.real == float(exact) and got.imag == 0.0, (n, got, exact)
+
+
+def test_c2b_bernoulli26_exact():
+    """Config 2b: n=26 Bernoulli(1/2) 0/1 matrices; the exact 128-bit result
+    equals the oracle's integer permanent, and the double result is its
+    correctly rounded value."""
+    d = np.load(GOLDEN / "bernoulli26.npz")
+    for k in range(d["mats"].shape[0]):
+        A = d["mats"][k].astype(float)
+        exact = int(str(d["perm"][k]))
+        v, dv = C.permanent_exact(A)
+        assert v == exact
+        assert dv == float(exact)
+        got = BSU.BBFGPermanentDouble(A).calculate()
+        assert got.real == float(exact)
+
+
+def derangements(n):
+    """!n by the exact recurrence !n = (n-1) (!(n-1) + !(n-2))."""
+    a, b = 1, 0  # !0, !1
+    if n == 0:
+        return 1
+    for k in range(2, n + 1):
+        a, b = b, (k - 1) * (a + b)
+    return b
+
+
+def test_exact_derangements():
+    for n in (10, 16, 20, 25):
+        D = np.ones((n, n)) - np.eye(n)
+        v, d = C.permanent_exact(D)
+        assert v == derangements(n), n
+        assert d == float(derangements(n))
+
+
+def test_repeated_kat(kat_cases):
+    """unittests/test_permanent.py through every repeated calculator class."""
+    classes = (BSU.BBFGRepeatedPermanentCalculatorDouble, BSU.BBFGRepeatedPermanentCalculatorLongDouble,
+               BSU.GlynnRepeatedPermanentCalculator, BSU.GlynnRepeatedPermanentCalculatorDouble,
+               BSU.ChinHuhPermanentCalculator, BSU.GlynnRepeatedInfPermanentCalculator)
+    for c in kat_cases:
+        for cls in classes:
+            v = cls(c["matrix"], c["input"], c["output"]).calculate()
+            assert rel_err(v, c["value"]) < 1e-12, (c["name"], cls.__name__, v)
+
+
+def test_connector_kat(kat_cases):
+    from piquassoboost_amd.connector import cpp_permanent_function
+
+    for c in kat_cases:
+        assert rel_err(cpp_permanent_function(c["matrix"], c["input"], c["output"]), c["value"]) < 1e-12
+
+
+def test_repeated_list_batched():
+    rng = np.random.default_rng(31)
+    U = random_complex(rng, 5, 5, scale=0.4)
+    ins = [np.array([1, 1, 0, 1, 0]), np.array([2, 0, 0, 1, 0])]
+    outs = [[np.array([0, 1, 1, 1, 0]), np.array([3, 0, 0, 0, 0])], [np.array([1, 0, 0, 0, 2])]]
+    res = BSU.BBFGRepeatedPermanentCalculatorDouble(U, ins, outs).calculate()
+    assert len(res) == 2 and len(res[0]) == 2 and len(res[1]) == 1
+    for i in range(2):
+        for j, o in enumerate(outs[i]):
+            assert rel_err(res[i][j], oracle.bbfg_repeated(U, ins[i], o, "dd")) < 1e-12
