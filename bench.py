@@ -70,7 +70,20 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
+                         "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
+    if a.copy_mb:
+        import torch
+
+        src = torch.ones(a.copy_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 10}), flush=True)
+        return 0
     if a.gpus != 1:
         print("bench.py measures one GPU: the permanent engine runs on a single MI355X in this build "
               "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)

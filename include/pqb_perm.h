@@ -61,7 +61,7 @@ extern "C" {
  * PQB_PREC_FP64     Force the FP64 sweep, never the integer route.
  * PQB_PREC_EXACT    Integer sweep modulo 2^128; fails with PQB_ERR_EXACT
  *                   unless every entry is an integer with zero imaginary part
- *                   and prod_i sum_j |a_ij| * 2^(rows-1) < 2^127.            */
+ *                   and pqb_exact_bound() proves 2^(rows-1) |perm| < 2^126.5. */
 #define PQB_PREC_DOUBLE 0
 #define PQB_PREC_EXTENDED 1
 #define PQB_PREC_FP64 2
@@ -104,6 +104,15 @@ int pqb_perm(const double* A, int64_t rows, int64_t cols, int64_t ld, int algo,
  * BBFGPermanentCalculator.cpp:100-107). */
 int pqb_perm_exact(const double* A, int64_t rows, int64_t cols, int64_t ld,
                    int64_t* words, double* out);
+
+/* Host-only eligibility check of the exact integer route (no device needed):
+ * PQB_OK with log2_bound = log2 of a proven upper bound on
+ * 2^(rows-1) |perm(A)| when the matrix is real and integer valued and the
+ * bound is below 2^126.5; PQB_ERR_EXACT otherwise.  Bounds used:
+ * |perm(A)| <= min(prod of row abs sums, prod of column abs sums), and
+ * Bregman's bound prod_i (r_i!)^(1/r_i) for square matrices with entries in
+ * {-1, 0, 1}.  No reference counterpart. */
+int pqb_exact_bound(const double* A, int64_t rows, int64_t cols, int64_t ld, double* log2_bound);
 
 /* ---- repeated-row permanent -----------------------------------------------
  * Replaces BBFGPermanentCalculatorRepeated::calculate(matrix&,

@@ -45,6 +45,7 @@ SIGNATURES = {
     "pqb_perm": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
                                 ctypes.c_int, _dp]),
     "pqb_perm_exact": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _dp]),
+    "pqb_exact_bound": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp]),
     "pqb_perm_repeated": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
                                          ctypes.c_int, ctypes.c_int, _dp]),
     "pqb_perm_device_workspace": (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
@@ -144,6 +145,18 @@ def permanent_repeated(A, row_mult, col_mult, algo: int = ALGO_BBFG, precision: 
     _check(lib().pqb_perm_repeated(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
                                    r.ctypes.data_as(_i64p), c.ctypes.data_as(_i64p), algo, precision, out))
     return complex(out[0], out[1])
+
+
+def exact_bound(A) -> tuple[bool, float]:
+    """(eligible, log2 bound) of the exact integer route; host only."""
+    M = as_matrix(A)
+    b = ctypes.c_double(0.0)
+    rc = lib().pqb_exact_bound(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1], ctypes.byref(b))
+    if rc == PQB_ERR_EXACT and b.value == 0.0 and "integer-valued" in lib().pqb_last_error().decode():
+        return False, float("nan")
+    if rc not in (PQB_OK, PQB_ERR_EXACT):
+        _check(rc)
+    return rc == PQB_OK, b.value
 
 
 def plain_plan(rows: int, cols: int, lanes_resident: int = 0) -> dict:

@@ -158,32 +158,48 @@ PlainPlan make_plain_plan(int R, int NC, int64_t lanes_resident) {
     return pl;
 }
 
-// Column abs-sum bound: |sum_delta ...| <= 2^(R-1) prod_j sum_i |a_ij|.
-// Returns true and fills `ai` (packed R x C int32) when the matrix is
-// integer valued, real, and the exact sweep provably cannot overflow.
-static bool integer_matrix(const double* A, int64_t R, int64_t C, int64_t ld,
-                           std::vector<int>& ai, bool* is_integer) {
-    *is_integer = false;
+// Integer route eligibility.  Fills `ai` (packed R x C int32) when the matrix
+// is integer valued and real, and returns log2 of an upper bound on
+// |sum_delta ...| = 2^(R-1) |perm(A)| (or -inf when perm(A) is provably 0):
+//   |perm(A)| <= perm(|A|) <= min(prod_i r_i, prod_j c_j)  (row / column abs sums)
+// and, for entries in {-1, 0, 1}, Bregman's bound perm(|A|) <= prod_i (r_i!)^(1/r_i)
+// (also applied to the transpose).  The sweep sums modulo 2^128, so the result
+// is exact when the bound is below 2^127; column sums must also fit in int32.
+static bool integer_matrix(const double* A, int64_t R, int64_t C, int64_t ld, std::vector<int>& ai,
+                           long double* log2_bound) {
     ai.assign((size_t)(R * C), 0);
-    std::vector<double> colabs((size_t)C, 0.0);
+    std::vector<double> rowabs((size_t)R, 0.0), colabs((size_t)C, 0.0);
+    bool unit = true;
     for (int64_t i = 0; i < R; ++i) {
         for (int64_t j = 0; j < C; ++j) {
             const double re = A[2 * (i * ld + j)];
             const double im = A[2 * (i * ld + j) + 1];
             if (im != 0.0 || !(std::fabs(re) < 1073741824.0) || re != std::floor(re)) return false;
             ai[(size_t)(i * C + j)] = (int)re;
-            colabs[(size_t)j] += std::fabs(re);
+            const double a = std::fabs(re);
+            if (a > 1.0) unit = false;
+            rowabs[(size_t)i] += a;
+            colabs[(size_t)j] += a;
         }
     }
-    *is_integer = true;
-    long double lg = (long double)(R - 1);
-    for (int64_t j = 0; j < C; ++j) {
+    for (int64_t j = 0; j < C; ++j)
         if (colabs[(size_t)j] >= 1073741824.0) return false;  // int32 column sums
-        if (colabs[(size_t)j] == 0.0) return true;             // zero column: perm = 0, no overflow
-        lg += std::log2((long double)colabs[(size_t)j]);
-    }
-    return lg < 126.0L;
+    auto lg_prod = [](const std::vector<double>& v, bool bregman) -> long double {
+        long double s = 0.0L;
+        for (double x : v) {
+            if (x == 0.0) return -INFINITY;
+            s += bregman ? std::lgamma((long double)x + 1.0L) / (long double)x / std::log(2.0L)
+                         : std::log2((long double)x);
+        }
+        return s;
+    };
+    long double lb = std::min(lg_prod(rowabs, false), lg_prod(colabs, false));
+    if (unit && R == C) lb = std::min(lb, std::min(lg_prod(rowabs, true), lg_prod(colabs, true)));
+    *log2_bound = lb + (long double)(R - 1);
+    return true;
 }
+
+static bool exact_fits(long double log2_bound) { return log2_bound < 126.5L; }
 
 typedef __int128 i128;
 
@@ -323,9 +339,9 @@ int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int alg
     const int R = (int)rows, C = (int)cols;
 
     std::vector<int> ai;
-    bool is_int = false;
-    const bool exact_ok =
-        precision != PQB_PREC_FP64 && integer_matrix(A, rows, cols, ld, ai, &is_int);
+    long double lb = 0.0L;
+    const bool is_int = precision != PQB_PREC_FP64 && integer_matrix(A, rows, cols, ld, ai, &lb);
+    const bool exact_ok = is_int && exact_fits(lb);
     if (precision == PQB_PREC_EXACT && !exact_ok)
         return fail(PQB_ERR_EXACT, is_int ? "exact mode: the permanent may exceed the 128-bit range"
                                           : "exact mode needs a real integer-valued matrix");
@@ -445,6 +461,16 @@ int pqb_perm_repeated(const double* A, int64_t rows, int64_t cols, int64_t ld, c
     } catch (const std::exception& ex) {
         return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
     }
+}
+
+int pqb_exact_bound(const double* A, int64_t rows, int64_t cols, int64_t ld, double* log2_bound) {
+    if (!A || !log2_bound || rows < 1 || cols < 1 || ld < cols) return fail(PQB_ERR_ARG, "bad arguments");
+    std::vector<int> ai;
+    long double lb = 0.0L;
+    if (!integer_matrix(A, rows, cols, ld, ai, &lb))
+        return fail(PQB_ERR_EXACT, "exact mode needs a real integer-valued matrix");
+    *log2_bound = (double)lb;
+    return exact_fits(lb) ? PQB_OK : fail(PQB_ERR_EXACT, "exact mode: the permanent may exceed the 128-bit range");
 }
 
 size_t pqb_perm_device_workspace(int64_t rows, int64_t cols) {

@@ -132,3 +132,25 @@ def test_plan_is_host_only():
         q = C.plain_plan(n, n, lanes_resident=196608)
         assert q["K"] + q["H"] == n - 1
         assert 0 <= q["K"] <= 14
+
+
+def test_exact_route_bounds():
+    """Which integer matrices the exact 128-bit sweep accepts (host-only check)."""
+    import math
+
+    # J_26: prod bound 26^26 is too weak, Bregman's bound (26!) proves 2^25 * 26! < 2^127
+    ok, lb = C.exact_bound(np.ones((26, 26)))
+    assert ok and abs(lb - (math.log2(math.factorial(26)) + 25)) < 1e-9
+    # derangement matrices up to n = 26
+    for n in (10, 20, 25, 26):
+        ok, _ = C.exact_bound(np.ones((n, n)) - np.eye(n))
+        assert ok, n
+    # a row of zeros: permanent provably 0
+    Z = np.ones((5, 5))
+    Z[2] = 0
+    assert C.exact_bound(Z)[0]
+    # large entries overflow the bound
+    assert not C.exact_bound(np.full((20, 20), 1000.0))[0]
+    # non-integer or complex input is not eligible
+    assert not C.exact_bound(np.array([[0.5, 1.0], [1.0, 1.0]]))[0]
+    assert not C.exact_bound(np.array([[1j, 1.0], [1.0, 1.0]]))[0]
