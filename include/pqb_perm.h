@@ -129,6 +129,42 @@ int pqb_perm_repeated(const double* A, int64_t rows, int64_t cols, int64_t ld,
                       const int64_t* row_mult, const int64_t* col_mult, int algo,
                       int precision, double* out);
 
+/* ---- batched repeated-row permanents ---------------------------------------
+ * One task = the arguments of one pqb_perm_repeated call.  The whole batch
+ * runs as one graph-captured launch of a persistent kernel (plus a small
+ * reduction) whatever the mix of sizes; out receives 2 doubles per task.
+ * Replaces the per-pair loop of BatchednPermanentCalculator::calculate
+ * (piquassoboost/sampling/source/BatchedPermanentCalculator.cpp:109-296) and
+ * the list-batched surfaces of the Python wrappers
+ * (GlynnPermanentCalculator_Wrapper.hpp:204-293,
+ * ChinHuhPermanentCalculator_Wrapper.hpp:233-323).  A failing task fails the
+ * batch; the message names the task index. */
+typedef struct pqb_task {
+    const double* A;          /* host, row-major complex128 */
+    int64_t rows, cols, ld;
+    const int64_t* row_mult;  /* len rows (output state) */
+    const int64_t* col_mult;  /* len cols (input state)  */
+} pqb_task;
+
+int pqb_perm_batch(const pqb_task* tasks, int64_t ntasks, int algo, int precision, double* out);
+
+/* pqb_perm_batch with an explicit walk length (Gray steps per chunk) instead
+ * of the built-in choice; used by the autotuner.  No reference counterpart. */
+int pqb_perm_batch_tuned(const pqb_task* tasks, int64_t ntasks, int algo, int precision,
+                         int64_t walk_length, double* out);
+
+/* Replays the graph of the last batch `iters` times on the engine stream with
+ * its inputs already resident and reports the elapsed milliseconds between
+ * HIP events around the replays.  No reference counterpart (measurement). */
+int pqb_batch_replay(int iters, double* ms);
+
+/* Host-only plan of one repeated task (no device needed): info[0] = column
+ * slots, [1] = rows incl. anchor, [2] = low digits, [3] = high digits,
+ * [4] = walk length, [5] = chunks.  All zero when no sweep is needed. */
+int pqb_repeated_plan(const double* A, int64_t rows, int64_t cols, int64_t ld,
+                      const int64_t* row_mult, const int64_t* col_mult,
+                      int64_t walk_length, int64_t* info);
+
 /* ---- device-resident plain permanent (inputs already in HBM) -------------
  * dA: device pointer to a packed rows x cols complex128 matrix (ld = cols).
  * d_out: device pointer to 6 doubles: [re, im, hi_re, lo_re, hi_im, lo_im]

@@ -48,6 +48,12 @@ SIGNATURES = {
     "pqb_exact_bound": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp]),
     "pqb_perm_repeated": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
                                          ctypes.c_int, ctypes.c_int, _dp]),
+    "pqb_perm_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp]),
+    "pqb_perm_batch_tuned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int64, _dp]),
+    "pqb_batch_replay": (ctypes.c_int, [ctypes.c_int, _dp]),
+    "pqb_repeated_plan": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
+                                         ctypes.c_int64, _i64p]),
     "pqb_perm_device_workspace": (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
     "pqb_perm_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
@@ -145,6 +151,66 @@ def permanent_repeated(A, row_mult, col_mult, algo: int = ALGO_BBFG, precision: 
     _check(lib().pqb_perm_repeated(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
                                    r.ctypes.data_as(_i64p), c.ctypes.data_as(_i64p), algo, precision, out))
     return complex(out[0], out[1])
+
+
+class _Task(ctypes.Structure):
+    """struct pqb_task of include/pqb_perm.h"""
+
+    _fields_ = [("A", _dp), ("rows", ctypes.c_int64), ("cols", ctypes.c_int64), ("ld", ctypes.c_int64),
+                ("row_mult", _i64p), ("col_mult", _i64p)]
+
+
+def _task_array(tasks):
+    """tasks: iterable of (matrix, row_mult, col_mult).  Returns the ctypes array
+    and the numpy arrays it points into (kept alive by the caller)."""
+    keep = []
+    arr = (_Task * len(tasks))()
+    for i, (A, r, c) in enumerate(tasks):
+        M = as_matrix(A)
+        rv = as_state(r)
+        cv = as_state(c)
+        if rv.shape[0] != M.shape[0] or cv.shape[0] != M.shape[1]:
+            raise Exception(
+                f"task {i}: state lengths must match the matrix: len(output_state) == rows, len(input_state) == cols"
+            )
+        keep += [M, rv, cv]
+        arr[i] = _Task(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1], rv.ctypes.data_as(_i64p),
+                       cv.ctypes.data_as(_i64p))
+    return arr, keep
+
+
+def permanent_batch(tasks, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE, walk_length: int = 0) -> list:
+    """Permanents of many repeated-row tasks (matrix, row_mult, col_mult) in one launch."""
+    tasks = list(tasks)
+    if not tasks:
+        return []
+    arr, keep = _task_array(tasks)
+    out = (ctypes.c_double * (2 * len(tasks)))()
+    if walk_length:
+        _check(lib().pqb_perm_batch_tuned(ctypes.cast(arr, ctypes.c_void_p), len(tasks), algo, precision,
+                                          walk_length, out))
+    else:
+        _check(lib().pqb_perm_batch(ctypes.cast(arr, ctypes.c_void_p), len(tasks), algo, precision, out))
+    del keep
+    return [complex(out[2 * i], out[2 * i + 1]) for i in range(len(tasks))]
+
+
+def batch_replay_ms(iters: int) -> float:
+    """Replay the last batch's graph `iters` times (inputs resident); elapsed ms."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().pqb_batch_replay(iters, ctypes.byref(ms)))
+    return ms.value
+
+
+def repeated_plan(A, row_mult, col_mult, walk_length: int) -> dict:
+    """Host-only decomposition of one repeated task (no device needed)."""
+    M = as_matrix(A)
+    r = as_state(row_mult)
+    c = as_state(col_mult)
+    info = (ctypes.c_int64 * 6)()
+    _check(lib().pqb_repeated_plan(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
+                                   r.ctypes.data_as(_i64p), c.ctypes.data_as(_i64p), walk_length, info))
+    return dict(zip(["slots", "rows", "low", "high", "walk", "chunks"], [int(v) for v in info]))
 
 
 def exact_bound(A) -> tuple[bool, float]:

@@ -1,0 +1,645 @@
+// Host side of the repeated-row and batched sweeps: task planning, step
+// tables, work items and the graph-captured persistent launch.
+//
+// Planning a task (one repeated-row permanent), following
+// BBFGPermanentCalculatorRepeated.hpp:74-101 and :116-307:
+//   * rows / columns with zero multiplicity are dropped (they contribute a
+//     factor 1, BatchedPermanentCalculator.cpp:412-451 builds the same compact
+//     matrix);
+//   * perm(A) = perm(A^T), so the orientation with fewer Gray steps
+//     S = m_min prod_{i != min} (m_i + 1) is used (the DFE host code makes the
+//     same choice, GlynnPermanentCalculatorRepeatedDFE.cpp:1138);
+//   * the row with the smallest multiplicity (first occurrence) becomes the
+//     anchor with multiplicity 1 and the rest of its multiplicity stays a digit
+//     (BBFGPermanentCalculatorRepeated.hpp:78-94);
+//   * columns of multiplicity c occupy c slots;
+//   * digits are split into low digits (walked in a chunk, radix product L close
+//     to the requested chunk length) and high digits (chunk index).
+// The low walk is a reflected mixed-radix Gray code starting at k = 0: each
+// digit sweeps up then down while the faster digits complete full sweeps, so
+// every step moves exactly one digit by one; tables are shared by all tasks
+// whose low radices agree.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pqb_perm.h"
+#include "pqb_device.h"
+#include "pqb_engine.h"
+#include "pqb_kernels.h"
+
+namespace pqb {
+
+// ----------------------------------------------------------------- planning --
+
+static uint64_t binom_u64(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    uint64_t c = 1;
+    for (int i = 1; i <= k; ++i) c = c * (uint64_t)(n - k + i) / (uint64_t)i;  // exact at every step
+    return c;
+}
+
+// Gray steps of the repeated sweep for multiplicities v (nonzero entries).
+static long double sweep_steps(const std::vector<int64_t>& v) {
+    if (v.empty()) return 0.0L;
+    size_t mi = 0;
+    for (size_t i = 1; i < v.size(); ++i)
+        if (v[i] < v[mi]) mi = i;
+    long double s = (long double)v[mi];
+    for (size_t i = 0; i < v.size(); ++i)
+        if (i != mi) s *= (long double)(v[i] + 1);
+    return s;
+}
+
+int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* rmult,
+              const int64_t* cmult, uint64_t target_L, TaskPlan& P) {
+    P = TaskPlan();
+    std::vector<int64_t> ri, rm, ci, cm;
+    int64_t N = 0, Nc = 0;
+    for (int64_t i = 0; i < rows; ++i)
+        if (rmult[i] > 0) {
+            ri.push_back(i);
+            rm.push_back(rmult[i]);
+            N += rmult[i];
+        }
+    for (int64_t j = 0; j < cols; ++j)
+        if (cmult[j] > 0) {
+            ci.push_back(j);
+            cm.push_back(cmult[j]);
+            Nc += cmult[j];
+        }
+    if (N != Nc) return fail(PQB_ERR_STATE, "Number of input and output states should be equal");
+    if (N == 0) {
+        P.trivial = true;
+        return PQB_OK;
+    }
+    if (N > kMaxSlots) return fail(PQB_ERR_SIZE, "more than " + std::to_string(kMaxSlots) + " photons");
+    const bool transpose = sweep_steps(cm) < sweep_steps(rm);
+    // element (r, c) of the oriented compact matrix
+    auto elem = [&](int64_t r, int64_t c, double* re, double* im) {
+        // oriented row r / column c -> original row i, column j
+        const int64_t i = transpose ? ri[(size_t)c] : ri[(size_t)r];
+        const int64_t j = transpose ? ci[(size_t)r] : ci[(size_t)c];
+        *re = A[2 * (i * ld + j)];
+        *im = A[2 * (i * ld + j) + 1];
+    };
+    const std::vector<int64_t>& row_m = transpose ? cm : rm;
+    const std::vector<int64_t>& col_m = transpose ? rm : cm;
+
+    // column slots
+    std::vector<int64_t> slot_col;
+    for (size_t c = 0; c < col_m.size(); ++c)
+        for (int64_t k = 0; k < col_m[c]; ++k) slot_col.push_back((int64_t)c);
+    const int NS = (int)slot_col.size();
+
+    // anchor: smallest multiplicity, first occurrence
+    size_t anchor = 0;
+    for (size_t r = 1; r < row_m.size(); ++r)
+        if (row_m[r] < row_m[anchor]) anchor = r;
+    struct Digit { int64_t src; int m; };
+    std::vector<Digit> digits;
+    for (size_t r = 0; r < row_m.size(); ++r) {
+        const int64_t m = (r == anchor) ? row_m[r] - 1 : row_m[r];
+        if (m > 0) digits.push_back({(int64_t)r, (int)m});
+    }
+    // low / high split: largest radices first into the walk while it fits
+    std::vector<size_t> ord(digits.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return digits[a].m > digits[b].m; });
+    uint64_t L = 1;
+    for (;;) {
+        std::vector<Digit> low, high;
+        L = 1;
+        for (size_t k : ord) {
+            const uint64_t rad = (uint64_t)digits[k].m + 1;
+            if (L * rad <= target_L) {
+                low.push_back(digits[k]);
+                L *= rad;
+            } else {
+                high.push_back(digits[k]);
+            }
+        }
+        long double nch = 1.0L;
+        for (const Digit& d : high) nch *= (long double)(d.m + 1);
+        if (nch < 4.0e9L) {
+            P.nchunks = (uint64_t)nch;
+            // rows of A: anchor, low digits in walk order, high digits
+            std::vector<Digit> order_rows;
+            order_rows.push_back({(int64_t)anchor, 1});
+            for (const Digit& d : low) order_rows.push_back(d);
+            for (const Digit& d : high) order_rows.push_back(d);
+            P.R = (int)order_rows.size();
+            P.NS = NS;
+            P.N = (int)N;
+            P.nlow = (int)low.size();
+            P.nhigh = (int)high.size();
+            P.L = L;
+            P.A.assign((size_t)P.R * NS * 2, 0.0);
+            for (int r = 0; r < P.R; ++r)
+                for (int s = 0; s < NS; ++s) {
+                    double re, im;
+                    elem(order_rows[(size_t)r].src, slot_col[(size_t)s], &re, &im);
+                    P.A[2 * ((size_t)r * NS + s)] = re;
+                    P.A[2 * ((size_t)r * NS + s) + 1] = im;
+                }
+            P.base.assign((size_t)NS * 2, 0.0);
+            for (int s = 0; s < NS; ++s) {
+                // anchor + sum_low m a: in the order of the reference seed
+                // (BBFGPermanentCalculatorRepeated.hpp:202-226)
+                double br = P.A[2 * (size_t)s], bi = P.A[2 * (size_t)s + 1];
+                for (int d = 0; d < P.nlow; ++d) {
+                    const double m = (double)low[(size_t)d].m;
+                    br += P.A[2 * ((size_t)(1 + d) * NS + s)] * m;
+                    bi += P.A[2 * ((size_t)(1 + d) * NS + s) + 1] * m;
+                }
+                P.base[2 * (size_t)s] = br;
+                P.base[2 * (size_t)s + 1] = bi;
+            }
+            P.low_radix.clear();
+            for (const Digit& d : low) P.low_radix.push_back(d.m + 1);
+            P.hrad.clear();
+            for (const Digit& d : high) P.hrad.push_back((uint32_t)(d.m + 1));
+            P.scale = std::ldexp(1.0, -(int)(N - 1));
+            return PQB_OK;
+        }
+        target_L *= 4;  // too many chunks for 32-bit chunk indices: walk more per chunk
+    }
+}
+
+// Reflected mixed-radix Gray walk from k = 0 over digits with the given radices
+// (digit 0 moves fastest).  Entry t describes the move into index t.
+void build_table(const std::vector<int>& radix, std::vector<RepStep>& tab) {
+    uint64_t L = 1;
+    for (int r : radix) L *= (uint64_t)r;
+    tab.assign((size_t)L, RepStep{0u, 0u, 1.0});
+    const size_t n = radix.size();
+    std::vector<int> g(n, 0), o(n, 1);
+    for (uint64_t t = 1; t < L; ++t) {
+        size_t d = 0;
+        int nv = 0;
+        for (;;) {
+            nv = g[d] + o[d];
+            if (nv >= 0 && nv <= radix[d] - 1) break;
+            o[d] = -o[d];
+            ++d;
+        }
+        const bool up = nv > g[d];
+        g[d] = nv;
+        int64_t w = 1;
+        int sum = 0;
+        for (size_t k = 0; k < n; ++k) {
+            w *= (int64_t)binom_u64(radix[k] - 1, g[k]);
+            sum += g[k];
+        }
+        tab[(size_t)t].row = (uint32_t)(1 + d);
+        tab[(size_t)t].dir = up ? 1u : 0u;
+        tab[(size_t)t].w = (double)((sum & 1) ? -w : w);
+    }
+}
+
+// ------------------------------------------------------------ batch buffers --
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct BatchBuffers {
+    void* dbuf = nullptr;
+    void* hbuf = nullptr;
+    size_t cap_tasks = 0, cap_items = 0, cap_data = 0;
+    size_t off_binom = 0, off_tasks = 0, off_items = 0, off_order = 0, off_data = 0, off_part = 0, off_out = 0,
+           total = 0;
+    hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int grid[5] = {0, 0, 0, 0, 0};
+    // last prepared batch (for replays)
+    int last_range = -1;
+    int last_ntasks = 0;
+
+    void layout(size_t nt, size_t ni, size_t nd) {
+        cap_tasks = nt;
+        cap_items = ni;
+        cap_data = nd;
+        off_binom = 256;
+        off_tasks = align256(off_binom + sizeof(double) * kBinomStride * kBinomStride);
+        off_items = align256(off_tasks + nt * sizeof(RepTask));
+        off_order = align256(off_items + ni * sizeof(RepItem));
+        off_data = align256(off_order + ni * sizeof(unsigned));
+        off_part = align256(off_data + nd);
+        off_out = align256(off_part + ni * sizeof(ddc));
+        total = align256(off_out + nt * 6 * sizeof(double));
+    }
+    void drop_graphs() {
+        for (auto& g : graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+};
+
+static BatchBuffers g_batch;
+
+static int ensure_batch_buffers_locked(size_t nt, size_t ni, size_t nd) {
+    BatchBuffers& B = g_batch;
+    if (B.dbuf && nt <= B.cap_tasks && ni <= B.cap_items && nd <= B.cap_data) return PQB_OK;
+    (void)hipStreamSynchronize(g_eng.stream);
+    B.drop_graphs();
+    if (B.dbuf) (void)hipFree(B.dbuf);
+    if (B.hbuf) (void)hipHostFree(B.hbuf);
+    B.dbuf = B.hbuf = nullptr;
+    B.layout(std::max<size_t>(nt, std::max<size_t>(2 * B.cap_tasks, 64)),
+             std::max<size_t>(ni, std::max<size_t>(2 * B.cap_items, 4096)),
+             std::max<size_t>(nd, std::max<size_t>(2 * B.cap_data, (size_t)1 << 20)));
+    hipError_t e = hipMalloc(&B.dbuf, B.total);
+    if (e != hipSuccess) {
+        B.dbuf = nullptr;
+        return hip_fail(e, "hipMalloc(batch)");
+    }
+    e = hipHostMalloc(&B.hbuf, B.off_data + B.cap_data, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        B.hbuf = nullptr;
+        return hip_fail(e, "hipHostMalloc(batch)");
+    }
+    std::memset(B.hbuf, 0, B.off_data);
+    double* bt = reinterpret_cast<double*>(static_cast<char*>(B.hbuf) + B.off_binom);
+    for (int m = 0; m < kBinomStride; ++m)
+        for (int k = 0; k < kBinomStride; ++k) bt[m * kBinomStride + k] = (double)binom_u64(m, k);
+    // counter + header start at zero on the device
+    e = hipMemcpyAsync(B.dbuf, B.hbuf, B.off_tasks, hipMemcpyHostToDevice, g_eng.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "batch init");
+    return PQB_OK;
+}
+
+static RepBatchParams batch_params() {
+    BatchBuffers& B = g_batch;
+    char* d = static_cast<char*>(B.dbuf);
+    RepBatchParams p;
+    p.tasks = reinterpret_cast<const RepTask*>(d + B.off_tasks);
+    p.items = reinterpret_cast<const RepItem*>(d + B.off_items);
+    p.order = reinterpret_cast<const unsigned*>(d + B.off_order);
+    p.binom = reinterpret_cast<const double*>(d + B.off_binom);
+    p.partials = reinterpret_cast<ddc*>(d + B.off_part);
+    p.out = reinterpret_cast<double*>(d + B.off_out);
+    p.head = reinterpret_cast<unsigned*>(d);
+    p.counts = reinterpret_cast<const unsigned*>(d + 64);
+    return p;
+}
+
+static int batch_grid_locked(int range) {
+    BatchBuffers& B = g_batch;
+    if (B.grid[range] == 0) {
+        int nb = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rep_batch_kernel_fn(range), 256, 0);
+        if (e != hipSuccess || nb <= 0) nb = 1;
+        B.grid[range] = nb * g_eng.cus;
+    }
+    return B.grid[range];
+}
+
+static int batch_graph_locked(int range, hipGraphExec_t* out) {
+    BatchBuffers& B = g_batch;
+    if (!B.graph[range]) {
+        const int grid = batch_grid_locked(range);
+        const RepBatchParams p = batch_params();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(g_eng.stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+        hipError_t le = launch_rep_batch(range, p, grid, g_eng.stream);
+        e = hipStreamEndCapture(g_eng.stream, &g);
+        if (le != hipSuccess) return hip_fail(le, "batch launch (capture)");
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        B.graph[range] = ex;
+    }
+    *out = B.graph[range];
+    return PQB_OK;
+}
+
+// Target walk length for a task of S steps when the batch holds `total`
+// steps: long walks amortise the per-chunk seed, but there must be enough
+// chunks to keep every resident lane busy.
+static uint64_t choose_L(long double S, long double total, int64_t lanes) {
+    long double want_chunks = std::max<long double>(4.0L * (long double)lanes * S / std::max(total, S), 1.0L);
+    long double L = S / want_chunks;
+    L = std::min<long double>(std::max<long double>(L, 16.0L), 2048.0L);
+    return (uint64_t)L;
+}
+
+// Run a list of planned tasks; results (6 doubles each) into out6.
+static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& out6) {
+    BatchBuffers& B = g_batch;
+    const size_t nt = plans.size();
+    out6.assign(nt * 6, 0.0);
+    // sweeps to run
+    std::vector<size_t> live;
+    int min_ns = 1 << 30, max_ns = 0;
+    for (size_t i = 0; i < nt; ++i) {
+        if (plans[i].trivial) {
+            out6[6 * i] = plans[i].tre;
+            out6[6 * i + 1] = plans[i].tim;
+            out6[6 * i + 2] = plans[i].tre;
+            out6[6 * i + 4] = plans[i].tim;
+            continue;
+        }
+        live.push_back(i);
+        min_ns = std::min(min_ns, plans[i].NS);
+        max_ns = std::max(max_ns, plans[i].NS);
+    }
+    if (live.empty()) return PQB_OK;
+    int range = rep_range_for(min_ns, max_ns);
+    if (range < 0) {
+        // split at 32 slots and run the two halves
+        std::vector<TaskPlan> lo, hi;
+        std::vector<size_t> ilo, ihi;
+        for (size_t i : live) {
+            if (plans[i].NS <= 32) {
+                lo.push_back(std::move(plans[i]));
+                ilo.push_back(i);
+            } else {
+                hi.push_back(std::move(plans[i]));
+                ihi.push_back(i);
+            }
+        }
+        std::vector<double> olo, ohi;
+        int st = run_tasks_locked(lo, olo);
+        if (st) return st;
+        st = run_tasks_locked(hi, ohi);
+        if (st) return st;
+        for (size_t k = 0; k < ilo.size(); ++k) std::copy(&olo[6 * k], &olo[6 * k] + 6, &out6[6 * ilo[k]]);
+        for (size_t k = 0; k < ihi.size(); ++k) std::copy(&ohi[6 * k], &ohi[6 * k] + 6, &out6[6 * ihi[k]]);
+        return PQB_OK;
+    }
+
+    // ---- work items ----
+    struct ItemCost { uint64_t cost; unsigned idx; };
+    std::vector<RepItem> items;
+    std::vector<ItemCost> costs;
+    std::vector<std::pair<uint32_t, uint32_t>> task_items(live.size());
+    const uint64_t lane_steps = 2048;  // target Gray steps per lane per item
+    for (size_t k = 0; k < live.size(); ++k) {
+        const TaskPlan& P = plans[live[k]];
+        const uint64_t per_lane = std::max<uint64_t>(1, lane_steps / P.L);
+        const uint64_t per_item = 64 * per_lane;
+        task_items[k].first = (uint32_t)items.size();
+        for (uint64_t c0 = 0; c0 < P.nchunks; c0 += per_item) {
+            const uint64_t cnt = std::min<uint64_t>(per_item, P.nchunks - c0);
+            costs.push_back({((cnt + 63) / 64) * P.L * (uint64_t)(P.NS + 4), (unsigned)items.size()});
+            items.push_back(RepItem{(uint32_t)k, (uint32_t)cnt, (uint32_t)c0, 0u});
+        }
+        task_items[k].second = (uint32_t)items.size() - task_items[k].first;
+    }
+    std::stable_sort(costs.begin(), costs.end(), [](const ItemCost& a, const ItemCost& b) { return a.cost > b.cost; });
+
+    // ---- data region: A, base per task, deduplicated step tables ----
+    std::map<std::vector<int>, size_t> table_off;
+    std::vector<std::pair<size_t, std::vector<RepStep>>> tables;  // (offset, table)
+    std::vector<size_t> a_off(live.size()), b_off(live.size()), t_off(live.size());
+    size_t nd = 0;
+    for (size_t k = 0; k < live.size(); ++k) {
+        const TaskPlan& P = plans[live[k]];
+        a_off[k] = nd;
+        nd = align256(nd + P.A.size() * sizeof(double));
+        b_off[k] = nd;
+        nd = align256(nd + P.base.size() * sizeof(double));
+        auto it = table_off.find(P.low_radix);
+        if (it == table_off.end()) {
+            tables.emplace_back(nd, std::vector<RepStep>());
+            build_table(P.low_radix, tables.back().second);
+            table_off[P.low_radix] = nd;
+            t_off[k] = nd;
+            nd = align256(nd + tables.back().second.size() * sizeof(RepStep));
+        } else {
+            t_off[k] = it->second;
+        }
+    }
+    int st = ensure_batch_buffers_locked(live.size(), items.size(), nd);
+    if (st) return st;
+    char* h = static_cast<char*>(B.hbuf);
+    char* d = static_cast<char*>(B.dbuf);
+    // header: head counter (0), counts
+    std::memset(h, 0, 256);
+    unsigned* counts = reinterpret_cast<unsigned*>(h + 64);
+    counts[0] = (unsigned)items.size();
+    counts[1] = (unsigned)live.size();
+    RepTask* T = reinterpret_cast<RepTask*>(h + B.off_tasks);
+    for (size_t k = 0; k < live.size(); ++k) {
+        const TaskPlan& P = plans[live[k]];
+        RepTask t;
+        std::memset(&t, 0, sizeof(t));
+        t.A = reinterpret_cast<const double2*>(d + B.off_data + a_off[k]);
+        t.base = reinterpret_cast<const double2*>(d + B.off_data + b_off[k]);
+        t.table = reinterpret_cast<const RepStep*>(d + B.off_data + t_off[k]);
+        t.NS = P.NS;
+        t.L = (int)P.L;
+        t.nhigh = P.nhigh;
+        t.item0 = task_items[k].first;
+        t.nitems = task_items[k].second;
+        t.out_index = (uint32_t)k;
+        t.nchunks = (uint32_t)P.nchunks;
+        t.scale = P.scale;
+        for (int hd = 0; hd < P.nhigh; ++hd) {
+            t.hrow[hd] = (uint32_t)(1 + P.nlow + hd);
+            t.hrad[hd] = P.hrad[(size_t)hd];
+        }
+        T[k] = t;
+        std::memcpy(h + B.off_data + a_off[k], P.A.data(), P.A.size() * sizeof(double));
+        std::memcpy(h + B.off_data + b_off[k], P.base.data(), P.base.size() * sizeof(double));
+    }
+    for (const auto& tb : tables)
+        std::memcpy(h + B.off_data + tb.first, tb.second.data(), tb.second.size() * sizeof(RepStep));
+    RepItem* I = reinterpret_cast<RepItem*>(h + B.off_items);
+    std::memcpy(I, items.data(), items.size() * sizeof(RepItem));
+    unsigned* O = reinterpret_cast<unsigned*>(h + B.off_order);
+    for (size_t k = 0; k < costs.size(); ++k) O[k] = costs[k].idx;
+
+    hipError_t e = hipMemcpyAsync(d, h, B.off_data + nd, hipMemcpyHostToDevice, g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "batch upload");
+    hipGraphExec_t gx = nullptr;
+    st = batch_graph_locked(range, &gx);
+    if (st) return st;
+    e = hipGraphLaunch(gx, g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    std::vector<double> res(live.size() * 6);
+    e = hipMemcpyAsync(res.data(), d + B.off_out, res.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       g_eng.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "batch sweep");
+    for (size_t k = 0; k < live.size(); ++k) std::copy(&res[6 * k], &res[6 * k] + 6, &out6[6 * live[k]]);
+    B.last_range = range;
+    B.last_ntasks = (int)live.size();
+    return PQB_OK;
+}
+
+// ------------------------------------------------------------- validation --
+
+// Reference rules for one repeated task (BBFGPermanentCalculatorRepeated.cpp:
+// 78-81, .hpp:119-128; GlynnPermanentCalculatorRepeated.hpp:55-63).  Returns
+// 1 with *decided set when no sweep is needed.
+static int check_task(const pqb_task& t, int algo, bool* decided, double* re) {
+    *decided = false;
+    if (t.rows < 0 || t.cols < 0) return fail(PQB_ERR_ARG, "negative matrix dimension");
+    if ((t.rows > 0 && !t.row_mult) || (t.cols > 0 && !t.col_mult))
+        return fail(PQB_ERR_ARG, "occupation pointer is NULL");
+    if (algo == PQB_ALGO_BBFG && t.rows == 0) {
+        *decided = true;
+        *re = 1.0;
+        return PQB_OK;
+    }
+    int64_t sr = 0, sc = 0;
+    for (int64_t i = 0; i < t.rows; ++i) {
+        if (t.row_mult[i] < 0) return fail(PQB_ERR_ARG, "negative occupation number");
+        sr += t.row_mult[i];
+    }
+    for (int64_t j = 0; j < t.cols; ++j) {
+        if (t.col_mult[j] < 0) return fail(PQB_ERR_ARG, "negative occupation number");
+        sc += t.col_mult[j];
+    }
+    if (sr != sc)
+        return fail(PQB_ERR_STATE,
+                    algo == PQB_ALGO_BBFG
+                        ? "BBFGPermanentCalculatorRepeated_Tasks::calculate:  Number of input and output states should be equal"
+                        : "GlynnPermanentCalculatorRepeated::calculate:  Number of input and output states should be equal");
+    if (t.rows == 0 || t.cols == 0 || sr == 0) {
+        *decided = true;
+        *re = 1.0;
+        return PQB_OK;
+    }
+    if (!t.A) return fail(PQB_ERR_ARG, "matrix pointer is NULL");
+    if (t.ld < t.cols) return fail(PQB_ERR_ARG, "leading dimension smaller than the number of columns");
+    if (sr > PQB_MAX_COLS) return fail(PQB_ERR_SIZE, "more than " + std::to_string(PQB_MAX_COLS) + " photons");
+    return PQB_OK;
+}
+
+int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, double* out, int64_t L_override) {
+    if (ntasks < 0) return fail(PQB_ERR_ARG, "negative task count");
+    if (ntasks > 0 && (!tasks || !out)) return fail(PQB_ERR_ARG, "NULL pointer");
+    if (algo != PQB_ALGO_BBFG && algo != PQB_ALGO_GLYNN) return fail(PQB_ERR_ARG, "unknown algorithm code");
+    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_EXACT) return fail(PQB_ERR_ARG, "unknown precision code");
+    if (precision == PQB_PREC_EXACT) return fail(PQB_ERR_ARG, "exact mode is available for the plain permanent only");
+    std::vector<TaskPlan> plans((size_t)ntasks);
+    std::vector<long double> steps((size_t)ntasks, 0.0L);
+    long double total = 0.0L;
+    for (int64_t i = 0; i < ntasks; ++i) {
+        bool decided = false;
+        double v = 1.0;
+        int st = check_task(tasks[i], algo, &decided, &v);
+        if (st) {
+            if (ntasks > 1) t_err = "task " + std::to_string(i) + ": " + t_err;
+            return st;
+        }
+        if (decided) {
+            plans[(size_t)i].trivial = true;
+            plans[(size_t)i].tre = v;
+            continue;
+        }
+        std::vector<int64_t> rm(tasks[i].row_mult, tasks[i].row_mult + tasks[i].rows);
+        std::vector<int64_t> cm(tasks[i].col_mult, tasks[i].col_mult + tasks[i].cols);
+        rm.erase(std::remove(rm.begin(), rm.end(), 0), rm.end());
+        cm.erase(std::remove(cm.begin(), cm.end(), 0), cm.end());
+        steps[(size_t)i] = std::min(sweep_steps(rm), sweep_steps(cm));
+        total += steps[(size_t)i];
+    }
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    int st = g_eng.ensure_init_locked(-1);
+    if (st) return st;
+    const int64_t lanes = (int64_t)batch_grid_locked(2) * 256;
+    for (int64_t i = 0; i < ntasks; ++i) {
+        if (plans[(size_t)i].trivial) continue;
+        const uint64_t L = L_override > 0 ? (uint64_t)L_override : choose_L(steps[(size_t)i], total, lanes);
+        st = plan_task(tasks[i].A, tasks[i].rows, tasks[i].cols, tasks[i].ld, tasks[i].row_mult, tasks[i].col_mult,
+                       L, plans[(size_t)i]);
+        if (st) return st;
+    }
+    std::vector<double> out6;
+    st = run_tasks_locked(plans, out6);
+    if (st) return st;
+    for (int64_t i = 0; i < ntasks; ++i) {
+        out[2 * i] = out6[(size_t)(6 * i)];
+        out[2 * i + 1] = out6[(size_t)(6 * i + 1)];
+    }
+    return PQB_OK;
+}
+
+int batch_replay(int iters, double* ms) {
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    BatchBuffers& B = g_batch;
+    if (B.last_range < 0 || !B.graph[B.last_range]) return fail(PQB_ERR_ARG, "no batch has been run yet");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, g_eng.stream);
+    for (int i = 0; e == hipSuccess && i < iters; ++i) e = hipGraphLaunch(B.graph[B.last_range], g_eng.stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, g_eng.stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float f = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "batch replay");
+    *ms = f;
+    return PQB_OK;
+}
+
+}  // namespace pqb
+
+using namespace pqb;
+
+extern "C" {
+
+int pqb_perm_batch(const pqb_task* tasks, int64_t ntasks, int algo, int precision, double* out) {
+    try {
+        return batch_entry(tasks, ntasks, algo, precision, out, 0);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+int pqb_perm_batch_tuned(const pqb_task* tasks, int64_t ntasks, int algo, int precision, int64_t walk_length,
+                         double* out) {
+    try {
+        return batch_entry(tasks, ntasks, algo, precision, out, walk_length);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+int pqb_batch_replay(int iters, double* ms) {
+    if (!ms || iters < 1) return fail(PQB_ERR_ARG, "bad replay arguments");
+    return batch_replay(iters, ms);
+}
+
+int pqb_repeated_plan(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* row_mult,
+                      const int64_t* col_mult, int64_t walk_length, int64_t* info) {
+    if (!info || walk_length < 1) return fail(PQB_ERR_ARG, "bad arguments");
+    pqb_task t{A, rows, cols, ld, row_mult, col_mult};
+    bool decided = false;
+    double v;
+    int st = check_task(t, PQB_ALGO_BBFG, &decided, &v);
+    if (st) return st;
+    if (decided) {
+        info[0] = info[1] = info[2] = info[3] = info[4] = info[5] = 0;
+        return PQB_OK;
+    }
+    TaskPlan P;
+    st = plan_task(A, rows, cols, ld, row_mult, col_mult, (uint64_t)walk_length, P);
+    if (st) return st;
+    info[0] = P.NS;
+    info[1] = P.R;
+    info[2] = P.nlow;
+    info[3] = P.nhigh;
+    info[4] = (int64_t)P.L;
+    info[5] = (int64_t)P.nchunks;
+    return PQB_OK;
+}
+
+}  // extern "C"

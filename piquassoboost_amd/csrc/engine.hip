@@ -364,54 +364,14 @@ int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int alg
     return run_plain_locked(A, R, C, ld, out);
 }
 
-// Repeated-row permanent: validation and edge rules of the reference, then
-// the sweep on the matrix with rows / columns repeated by multiplicity
-// (adaptInterferometerGlynnMultiplied, BatchedPermanentCalculator.cpp:367-399).
+// Repeated-row permanent: a batch of one task (batch.hip applies the
+// reference's validation and edge rules).
 int repeated_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* row_mult,
                    const int64_t* col_mult, int algo, int precision, double* out) {
     if (!out) return fail(PQB_ERR_ARG, "output pointer is NULL");
-    if (rows < 0 || cols < 0) return fail(PQB_ERR_ARG, "negative matrix dimension");
-    if (algo != PQB_ALGO_BBFG && algo != PQB_ALGO_GLYNN) return fail(PQB_ERR_ARG, "unknown algorithm code");
-    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_EXACT) return fail(PQB_ERR_ARG, "unknown precision code");
-    if ((rows > 0 && !row_mult) || (cols > 0 && !col_mult)) return fail(PQB_ERR_ARG, "occupation pointer is NULL");
-    if (algo == PQB_ALGO_BBFG && rows == 0) {  // BBFGPermanentCalculatorRepeated.cpp:78-81
-        out[0] = 1.0; out[1] = 0.0;
-        return PQB_OK;
-    }
-    int64_t sr = 0, sc = 0;
-    for (int64_t i = 0; i < rows; ++i) {
-        if (row_mult[i] < 0) return fail(PQB_ERR_ARG, "negative occupation number");
-        sr += row_mult[i];
-    }
-    for (int64_t j = 0; j < cols; ++j) {
-        if (col_mult[j] < 0) return fail(PQB_ERR_ARG, "negative occupation number");
-        sc += col_mult[j];
-    }
-    if (sr != sc)
-        return fail(PQB_ERR_STATE, algo == PQB_ALGO_BBFG
-            ? "BBFGPermanentCalculatorRepeated_Tasks::calculate:  Number of input and output states should be equal"
-            : "GlynnPermanentCalculatorRepeated::calculate:  Number of input and output states should be equal");
-    if (rows == 0 || cols == 0 || sr == 0) {  // BBFGPermanentCalculatorRepeated.hpp:126-128, GlynnPermanentCalculatorRepeated.hpp:62-63
-        out[0] = 1.0; out[1] = 0.0;
-        return PQB_OK;
-    }
-    if (!A) return fail(PQB_ERR_ARG, "matrix pointer is NULL");
-    if (ld < cols) return fail(PQB_ERR_ARG, "leading dimension smaller than the number of columns");
-    if (sr > PQB_MAX_COLS)
-        return fail(PQB_ERR_SIZE, "more than " + std::to_string(PQB_MAX_COLS) + " photons");
-    const int64_t N = sr;
-    std::vector<double> E((size_t)(2 * N * N));
-    int64_t r = 0;
-    for (int64_t i = 0; i < rows; ++i)
-        for (int64_t a = 0; a < row_mult[i]; ++a, ++r) {
-            int64_t c = 0;
-            for (int64_t j = 0; j < cols; ++j)
-                for (int64_t b = 0; b < col_mult[j]; ++b, ++c) {
-                    E[(size_t)(2 * (r * N + c))] = A[2 * (i * ld + j)];
-                    E[(size_t)(2 * (r * N + c) + 1)] = A[2 * (i * ld + j) + 1];
-                }
-        }
-    return plain_entry(E.data(), N, N, N, PQB_ALGO_BBFG, precision, out, nullptr);
+    if (precision == PQB_PREC_EXACT) return fail(PQB_ERR_ARG, "exact mode is available for the plain permanent only");
+    const pqb_task t{A, rows, cols, ld, row_mult, col_mult};
+    return batch_entry(&t, 1, algo, precision, out, 0);
 }
 
 }  // namespace pqb

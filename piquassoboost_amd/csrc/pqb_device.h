@@ -16,6 +16,23 @@ namespace pqb {
 
 constexpr int kWave = 64;
 
+// Read-only data reached through pointers loaded from memory (task tables,
+// step tables, matrices of a batch) is accessed through the constant address
+// space, so wave-uniform accesses become scalar loads with SGPR results.
+// (The host pass of hipcc parses device templates too; it sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+#else
+template <class T>
+using cptr = const T*;
+#endif
+
+template <class T>
+__device__ __forceinline__ cptr<T> to_const(const T* q) {
+    return (cptr<T>)q;
+}
+
 // ---------------------------------------------------------------- complex --
 
 // (ar + i ai) *= (br + i bi); four FP64 VALU instructions after contraction.

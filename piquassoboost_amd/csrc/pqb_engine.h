@@ -5,6 +5,7 @@
 
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "pqb_perm.h"
 
@@ -24,6 +25,28 @@ struct PlainPlan {
 };
 
 PlainPlan make_plain_plan(int R, int NC, int64_t lanes_resident);
+
+// One planned repeated-row permanent (see batch.hip).
+struct TaskPlan {
+    bool trivial = false;      // decided without a sweep: value (tre, tim)
+    double tre = 1.0, tim = 0.0;
+    int NS = 0;                // column slots
+    int R = 0;                 // rows of A: anchor, low digits, high digits
+    int N = 0;                 // photons
+    int nlow = 0, nhigh = 0;
+    uint64_t L = 1;            // walk length
+    uint64_t nchunks = 1;
+    double scale = 1.0;        // 2^-(N-1)
+    std::vector<double> A;     // R x NS complex (interleaved)
+    std::vector<double> base;  // NS complex
+    std::vector<int> low_radix;
+    std::vector<uint32_t> hrad;
+};
+
+int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* rmult,
+              const int64_t* cmult, uint64_t target_L, TaskPlan& P);
+int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, double* out,
+                int64_t L_override);
 
 struct Engine {
     std::mutex mu;

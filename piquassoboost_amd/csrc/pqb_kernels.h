@@ -32,6 +32,62 @@ struct ExactParams {
     unsigned long long* out;       // [lo, hi] of sum_delta (...) mod 2^128
 };
 
+// ---- repeated-row / batched sweep ----
+constexpr int kMaxDigits = 40;
+constexpr int kMaxSlots = 40;
+constexpr int kBinomStride = 41;  // binomial table: binom[m * 41 + k], 0 <= k <= m <= 40
+
+struct RepStep {          // one step of the low-digit Gray walk
+    uint32_t row;         // row of A whose digit moves (low digit d lives in row 1 + d)
+    uint32_t dir;         // 1: k += 1 (colsum -= 2a), 0: k -= 1 (colsum += 2a)
+    double w;             // prod_low (-1)^k C(m, k) after the step (exact integer)
+};
+
+struct RepTask {
+    const double2* A;      // R x NS: a_i (slots expanded); row 0 is the anchor, rows
+                           // 1..nlow the low digits (in walk order), then the high digits
+    const double2* base;   // NS: a_anchor + sum_{low i} m_i a_i  (seed of every chunk before high digits)
+    const RepStep* table;  // L entries, entry t = step into Gray index t (entry 0 unused)
+    int NS;                // column slots
+    int L;                 // low-walk length = prod_low (m_i + 1)
+    int nhigh;             // high digits
+    int pad0;
+    uint32_t item0;        // first work item of this task (items of a task are contiguous)
+    uint32_t nitems;
+    uint32_t out_index;    // result slot
+    uint32_t nchunks;      // prod_high (m_i + 1)
+    double scale;          // 2^-(N-1)
+    uint32_t hrow[kMaxDigits];  // row (in A) of high digit h
+    uint32_t hrad[kMaxDigits];  // radix m_h + 1 of high digit h (digit 0 varies fastest)
+};
+
+struct RepItem {
+    uint32_t task;
+    uint32_t count;   // chunks in the item
+    uint32_t chunk0;  // first chunk
+    uint32_t pad;
+};
+
+struct RepBatchParams {
+    const RepTask* tasks;
+    const RepItem* items;
+    const unsigned* order;  // processing order of the items (largest first)
+    const double* binom;    // binomial table
+    ddc* partials;          // one per item
+    double* out;            // 6 doubles per task result
+    unsigned* head;         // item counter, zero on entry; the reduce kernel resets it
+    const unsigned* counts; // device: counts[0] = items, counts[1] = tasks (read by the
+                            // kernels, so a captured graph serves every batch that fits)
+};
+
+constexpr int kReduceBlocks = 512;  // fixed grid of the per-task reduction (one wave per block)
+
+// Kernel instantiations cover slot ranges; rep_range_for() picks the narrowest
+// one for a batch (-1: the batch needs splitting at 32 slots).
+int rep_range_for(int min_ns, int max_ns);
+const void* rep_batch_kernel_fn(int range);
+hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks, hipStream_t st);
+
 hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, hipStream_t st);
 hipError_t launch_plain_prep(const double2* A, double2* M2, int n, hipStream_t st);
 hipError_t launch_plain_exact(int nc, const ExactParams& p, int nblocks, int bdim, hipStream_t st);

@@ -184,11 +184,18 @@ class ChinHuhPermanentCalculator_wrapper:
             # states.  With multiInput (ins[0] is a list): outs[i] is one output
             # state and ins[i] a list of input states.
             multi = len(ins) != 0 and isinstance(ins[0], list)
-            res = []
+            pairs = []  # (i, input state, output state)
             for i in range(len(ins)):
                 if multi:
-                    res.append([self._one(x, outs[i]) for x in ins[i]])
+                    pairs += [(i, C.as_state(x), C.as_state(outs[i])) for x in ins[i]]
                 else:
-                    res.append([self._one(ins[i], y) for y in outs[i]])
+                    pairs += [(i, C.as_state(ins[i]), C.as_state(y)) for y in outs[i]]
+            algo, prec = REPEATED_LIBS[self.lib]
+            # all pairs in one batched launch (the reference loops over them,
+            # ChinHuhPermanentCalculator_Wrapper.hpp:283-308)
+            vals = _run(C.permanent_batch, [(self._matrix, o, x) for _, x, o in pairs], algo, prec)
+            res = [[] for _ in range(len(ins))]
+            for (i, _, _), v in zip(pairs, vals):
+                res[i].append(v)
             return res
         return self._one(ins, outs)

@@ -154,3 +154,26 @@ def test_exact_route_bounds():
     # non-integer or complex input is not eligible
     assert not C.exact_bound(np.array([[0.5, 1.0], [1.0, 1.0]]))[0]
     assert not C.exact_bound(np.array([[1j, 1.0], [1.0, 1.0]]))[0]
+
+
+def test_repeated_plan_host_only():
+    """Task planning needs no device: anchor, orientation, digit split."""
+    U = np.eye(6, dtype=complex) + 0.1
+    # rows (output) multiplicities 1,3,0,2,1,2 ; columns all ones except a zero
+    outp = np.array([1, 3, 0, 2, 1, 2])
+    inp = np.array([2, 1, 1, 1, 2, 2])
+    p = C.repeated_plan(U, outp, inp, 1 << 20)
+    assert p["slots"] == 9
+    # whole walk inside one chunk
+    assert p["high"] == 0 and p["chunks"] == 1
+    # S is minimised over the orientation: rows give 1*4*3*2*3 = 72, cols 1*2*2*2*3*3 = 72 -> rows kept
+    assert p["walk"] * p["chunks"] == 72
+    p1 = C.repeated_plan(U, outp, inp, 1)
+    assert p1["low"] == 0 and p1["walk"] == 1 and p1["chunks"] == 72
+    # zero photons: no sweep
+    z = C.repeated_plan(U, np.zeros(6, np.int64), np.zeros(6, np.int64), 16)
+    assert z["chunks"] == 0
+    # transposed orientation when the columns repeat more
+    # 3 x 2 matrix, rows 1,1,1 (S = 1*2*2 = 4), columns 2,1 (S = 1*3 = 3): transposed
+    p2 = C.repeated_plan(np.ones((3, 2)), np.array([1, 1, 1]), np.array([2, 1]), 1 << 20)
+    assert p2["walk"] * p2["chunks"] == 3 and p2["slots"] == 3 and p2["rows"] == 2
