@@ -549,6 +549,15 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
         steps[(size_t)i] = std::min(sweep_steps(rm), sweep_steps(cm));
         total += steps[(size_t)i];
     }
+    bool any_sweep = false;
+    for (const TaskPlan& P : plans) any_sweep = any_sweep || !P.trivial;
+    if (!any_sweep) {  // every result decided by the edge rules: no device needed
+        for (int64_t i = 0; i < ntasks; ++i) {
+            out[2 * i] = plans[(size_t)i].tre;
+            out[2 * i + 1] = plans[(size_t)i].tim;
+        }
+        return PQB_OK;
+    }
     std::lock_guard<std::mutex> lk(g_eng.mu);
     int st = g_eng.ensure_init_locked(-1);
     if (st) return st;
