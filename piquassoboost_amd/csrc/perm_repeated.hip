@@ -195,10 +195,15 @@ __global__ __launch_bounds__(256) void rep_batch_kernel(RepBatchParams p) {
     const cptr<RepTask> tasks = to_const(p.tasks);
     const unsigned nitems = to_const(p.counts)[0];
     for (;;) {
-        unsigned idx = 0;
-        if (lane == 0) idx = atomicAdd(p.head, 1u);
-        idx = __builtin_amdgcn_readfirstlane(idx);  // wave-uniform from here on
-        if (idx >= nitems) break;
+        // Every lane executes the atomic (uniform control flow); only lane 0
+        // adds 1, and lane 0's old value is the wave's item.  A lane-0-only
+        // branch around the atomic must not be used here: the structurized
+        // loop can then re-enter the item fetch for the other lanes without
+        // the atomic, and they spin on a stale index.
+        const unsigned got = atomicAdd(p.head, lane == 0 ? 1u : 0u);
+        const unsigned idx0 = __builtin_amdgcn_readlane(got, 0);  // wave-uniform
+        if (idx0 >= nitems) break;
+        unsigned idx = idx0;
         idx = order[idx];  // largest items first; partials stay indexed by item
         const RepItem it = items[idx];
         const cptr<RepTask> T = tasks + it.task;
