@@ -181,10 +181,11 @@ int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
 
 /* ---- launch plan introspection (host only, no device needed) --------------
  * Writes the sweep decomposition chosen for a rows x cols plain permanent:
- * plan[0] = K (Gray bits per chunk), plan[1] = H (chunk-index bits),
- * plan[2] = launched lanes, plan[3] = threads per block, plan[4] = blocks.
- * `lanes_resident` = lanes the device keeps resident (0 = use the device). */
-int pqb_plain_plan(int64_t rows, int64_t cols, int64_t lanes_resident, int64_t* plan);
+ * plan[0] = K (Gray bits walked per group), plan[1] = B (lane bits),
+ * plan[2] = H (group bits), plan[3] = launched waves, plan[4] = threads per
+ * block, plan[5] = blocks.  `waves_resident` = waves the device keeps
+ * resident for this kernel (0 = ask the device). */
+int pqb_plain_plan(int64_t rows, int64_t cols, int64_t waves_resident, int64_t* plan);
 
 /* ---- timing helper ---------------------------------------------------------
  * Runs `iters` back-to-back pqb_perm_device launches on the engine's stream

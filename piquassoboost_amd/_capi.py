@@ -225,10 +225,10 @@ def exact_bound(A) -> tuple[bool, float]:
     return rc == PQB_OK, b.value
 
 
-def plain_plan(rows: int, cols: int, lanes_resident: int = 0) -> dict:
-    p = (ctypes.c_int64 * 5)()
-    _check(lib().pqb_plain_plan(rows, cols, lanes_resident, p))
-    return {"K": p[0], "H": p[1], "lanes": p[2], "block": p[3], "blocks": p[4]}
+def plain_plan(rows: int, cols: int, waves_resident: int = 0) -> dict:
+    p = (ctypes.c_int64 * 6)()
+    _check(lib().pqb_plain_plan(rows, cols, waves_resident, p))
+    return {"K": p[0], "B": p[1], "H": p[2], "waves": p[3], "block": p[4], "blocks": p[5]}
 
 
 def device_workspace_bytes(rows: int, cols: int) -> int:

@@ -115,7 +115,7 @@ int Engine::ensure_buffers_locked(size_t dneed, size_t hneed) {
     return PQB_OK;
 }
 
-int64_t Engine::resident_lanes_locked(int nc) {
+int64_t Engine::resident_waves_locked(int nc) {
     if (nc < 1 || nc > PQB_MAX_COLS) return 0;
     if (blocks_per_cu[nc] == 0) {
         int nb = 0;
@@ -123,7 +123,7 @@ int64_t Engine::resident_lanes_locked(int nc) {
         if (e != hipSuccess || nb <= 0) nb = 1;
         blocks_per_cu[nc] = nb;
     }
-    return (int64_t)blocks_per_cu[nc] * kBlock * cus;
+    return (int64_t)blocks_per_cu[nc] * (kBlock / kWave) * cus;
 }
 
 // --------------------------------------------------------------- planning --
@@ -137,24 +137,38 @@ static int floor_log2(int64_t v) {
     return r;
 }
 
-PlainPlan make_plain_plan(int R, int NC, int64_t lanes_resident) {
+static int ceil_log2(int64_t v) {
+    int r = 0;
+    while (((int64_t)1 << r) < v) ++r;
+    return r;
+}
+
+// Split of the R-1 sign bits into walk (K), lane (B) and group (H) bits and
+// the number of waves: about 32 groups per resident wave when the sweep is
+// large (runs differ by one group, so the imbalance stays near 3%), and the
+// shortest walk that still amortises a group transition otherwise.
+PlainPlan make_plain_plan(int R, int NC, int64_t waves_resident) {
+    (void)NC;
     PlainPlan pl;
-    const int T = R - 1;  // Gray bits
-    int64_t lr = std::max<int64_t>(lanes_resident, kWave);
-    const int lg = floor_log2(lr);  // lanes used: largest power of two <= resident
-    const int kmin = std::min(T, 4);
-    int K = T - lg - 2;  // about four chunks per lane when the problem is large
-    K = std::max(K, kmin);
-    K = std::min(K, std::min(T, 14));
+    const int T = R - 1;
+    const int64_t wr = std::max<int64_t>(waves_resident, 1);
+    int K = T - 6 - ceil_log2(wr * 32);
+    K = std::max(K, std::min(T, 3));
+    K = std::min(K, std::min(T, 12));
     pl.K = K;
-    pl.H = T - K;
-    const int64_t nchunks = (int64_t)1 << pl.H;
-    int64_t lanes = std::min<int64_t>(nchunks, (int64_t)1 << lg);
-    lanes = ((lanes + kWave - 1) / kWave) * kWave;
-    pl.block = (int)std::min<int64_t>(kBlock, lanes);
-    pl.blocks = (int)((lanes + pl.block - 1) / pl.block);
-    pl.lanes = (int64_t)pl.blocks * pl.block;
-    pl.nchunks = nchunks;
+    pl.B = std::min(6, T - K);
+    pl.H = T - K - pl.B;
+    const int64_t G = (int64_t)1 << pl.H;
+    int64_t waves = std::min<int64_t>(G, wr);
+    if (waves >= 4) {
+        waves -= waves % 4;
+        pl.block = kBlock;
+        pl.blocks = (int)(waves / 4);
+    } else {
+        pl.block = (int)(64 * waves);
+        pl.blocks = 1;
+    }
+    pl.waves = (int64_t)pl.blocks * pl.block / kWave;
     return pl;
 }
 
@@ -209,7 +223,7 @@ static double i128_to_double(i128 v) { return (double)v; }
 // then divide by 2^(R-1).  Caller holds the engine lock.
 static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* result) {
     const int T = R - 1;
-    int64_t lr = g_eng.resident_lanes_locked(C);
+    int64_t lr = g_eng.resident_waves_locked(C) * kWave;
     const int lg = floor_log2(std::max<int64_t>(lr, kWave));
     const int H = std::min(T, std::max(0, lg));
     const int K = T - H;
@@ -254,8 +268,7 @@ static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* resu
 
 // FP64 route.  Caller holds the engine lock.
 static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* out) {
-    const int64_t lr = g_eng.resident_lanes_locked(C);
-    const PlainPlan pl = make_plain_plan(R, C, lr);
+    const PlainPlan pl = make_plain_plan(R, C, g_eng.resident_waves_locked(C));
     const size_t m2_elems = (size_t)2 * R * C;
     const size_t m2_bytes = m2_elems * sizeof(double2);
     const size_t part_off = kCounterBytes;
@@ -282,8 +295,8 @@ static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* o
     p.M2 = reinterpret_cast<const double2*>(d + m2_off);
     p.R = R;
     p.K = pl.K;
+    p.B = pl.B;
     p.H = pl.H;
-    p.nchunks = pl.nchunks;
     p.partials = reinterpret_cast<ddc*>(d + part_off);
     p.counter = reinterpret_cast<unsigned*>(d);
     p.out = reinterpret_cast<double*>(d + out_off);
@@ -447,14 +460,14 @@ int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
         return fail(PQB_ERR_SIZE, "matrix has more than " + std::to_string(PQB_MAX_COLS) + " columns");
     if (work_bytes < pqb_perm_device_workspace(rows, cols)) return fail(PQB_ERR_ARG, "workspace too small");
     const int R = (int)rows, C = (int)cols;
-    int64_t lr;
+    int64_t wr;
     {
         std::lock_guard<std::mutex> lk(g_eng.mu);
         int st = g_eng.ensure_init_locked(-1);
         if (st) return st;
-        lr = g_eng.resident_lanes_locked(C);
+        wr = g_eng.resident_waves_locked(C);
     }
-    const PlainPlan pl = make_plain_plan(R, C, lr);
+    const PlainPlan pl = make_plain_plan(R, C, wr);
     if (pl.blocks > kMaxBlocks) return fail(PQB_ERR_SIZE, "launch plan exceeds the workspace layout");
     char* w = static_cast<char*>(d_work);
     double2* m2 = reinterpret_cast<double2*>(w + kCounterBytes + (size_t)kMaxBlocks * sizeof(ddc));
@@ -465,8 +478,8 @@ int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
     p.M2 = m2;
     p.R = R;
     p.K = pl.K;
+    p.B = pl.B;
     p.H = pl.H;
-    p.nchunks = pl.nchunks;
     p.partials = reinterpret_cast<ddc*>(w + kCounterBytes);
     p.counter = reinterpret_cast<unsigned*>(w);
     p.out = d_out;
@@ -476,23 +489,24 @@ int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
     return PQB_OK;
 }
 
-int pqb_plain_plan(int64_t rows, int64_t cols, int64_t lanes_resident, int64_t* plan) {
+int pqb_plain_plan(int64_t rows, int64_t cols, int64_t waves_resident, int64_t* plan) {
     if (!plan) return fail(PQB_ERR_ARG, "plan pointer is NULL");
     if (rows < 1 || cols < 1 || rows >= cols + 2 || cols > PQB_MAX_COLS)
         return fail(PQB_ERR_ARG, "no sweep for this shape");
-    int64_t lr = lanes_resident;
-    if (lr <= 0) {
+    int64_t wr = waves_resident;
+    if (wr <= 0) {
         std::lock_guard<std::mutex> lk(g_eng.mu);
         int st = g_eng.ensure_init_locked(-1);
         if (st) return st;
-        lr = g_eng.resident_lanes_locked((int)cols);
+        wr = g_eng.resident_waves_locked((int)cols);
     }
-    const PlainPlan pl = make_plain_plan((int)rows, (int)cols, lr);
+    const PlainPlan pl = make_plain_plan((int)rows, (int)cols, wr);
     plan[0] = pl.K;
-    plan[1] = pl.H;
-    plan[2] = pl.lanes;
-    plan[3] = pl.block;
-    plan[4] = pl.blocks;
+    plan[1] = pl.B;
+    plan[2] = pl.H;
+    plan[3] = pl.waves;
+    plan[4] = pl.block;
+    plan[5] = pl.blocks;
     return PQB_OK;
 }
 

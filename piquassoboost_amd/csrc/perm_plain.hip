@@ -10,17 +10,17 @@
 //
 // MI355X layout of the sweep (one permanent per launch):
 //
-// * The index space is split into 2^H chunks of 2^K consecutive Gray steps.
-//   A chunk is identified by its H high bits (rows 1 .. R-1-K); the K low bits
-//   (rows R-K .. R-1) are walked inside the chunk, always starting from
-//   delta = +1.  Every lane therefore walks the SAME K-bit reflected Gray
-//   sequence, so the flipped row and its direction are wave-uniform: the row
-//   of +/-2A is fetched with scalar loads and used as an SGPR operand, and the
-//   Gray bookkeeping (ctz, parity) runs on the SALU.
-// * A lane seeds its column sums directly from its chunk's high bits (like the
-//   reference's per-chunk seed, BBFGPermanentCalculator.hpp:143-167) and takes
-//   chunks g, g+L, g+2L, ... (L = launched lanes).  The seed costs 2*R*NC FMAs,
-//   about 1% of a 2^10-step chunk at NC=30.
+// * The R-1 sign bits are split into K walk bits (rows R-K .. R-1), B <= 6
+//   lane bits (rows R-K-B .. R-K-1) and H group bits (rows 1 .. R-1-K-B).
+//   Every lane of a wave walks the SAME K-bit reflected Gray sequence from
+//   delta = +1, so the flipped row and its direction are wave-uniform: the
+//   row of +/-2A is fetched with scalar loads and used as an SGPR operand,
+//   and the Gray bookkeeping (ctz, parity) runs on the SALU.
+// * Each wave owns a contiguous run of groups (runs differ by at most one
+//   group, so any number of resident waves is balanced).  It seeds once, like
+//   the reference's per-chunk seed (BBFGPermanentCalculator.hpp:143-167), and
+//   moves from group g to g+1 by flipping the rows of the changed group bits,
+//   which are again wave-uniform (about three row updates per group).
 // * The product of the NC column sums is formed in complex FP64 as four
 //   independent chains (for ILP) and the last multiply is fused into the
 //   accumulation (4 FMAs), giving 6*NC-4 VALU instructions per Gray step
@@ -112,15 +112,20 @@ __device__ __forceinline__ void add_row_signed(double (&cr)[NC], double (&ci)[NC
 }
 
 template <int NC>
-__global__ __launch_bounds__(256) void plain_sweep_kernel(PlainParams p) {
+// Up to 32 columns the sweep fits in 168 VGPRs: ask for three waves per SIMD
+// so that scalar-load and FP64 latencies of one wave hide behind the others.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 32 ? 3 : 1)))
+void plain_sweep_kernel(PlainParams p) {
     __shared__ ddc red[256 / kWave];
 
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t L = (int64_t)gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
 
     const int R = p.R;
-    const int K = p.K;
-    const int H = p.H;
+    const int K = p.K;  // walk bits:  rows R-K .. R-1
+    const int B = p.B;  // lane bits:  rows R-K-B .. R-K-1
+    const int H = p.H;  // group bits: rows 1 .. R-1-K-B
     const unsigned rowb = NC * sizeof(double2);
     // byte offsets of the +2A / -2A rows: row r of +2A at r*rowb, of -2A at (R+r)*rowb
     const char* __restrict__ Mb = reinterpret_cast<const char*>(p.M2);
@@ -128,12 +133,19 @@ __global__ __launch_bounds__(256) void plain_sweep_kernel(PlainParams p) {
     const unsigned low_p = (unsigned)(R - 1) * rowb;  // +2A[R-1]
     const unsigned low_m = neg + low_p;               // -2A[R-1]
     const unsigned W = 1u << K;
+    const bool lane_on = lane < (1 << B);
 
-    double tr = 0.0, ti = 0.0;  // lane total over its chunks
-    for (int64_t c = g; c < p.nchunks; c += L) {
-        // ---- seed: delta_0 = +1, low rows +1, high row R-1-K-h is -1 iff bit h of c.
-        // Built from the +2A rows and halved at the end (scaling by 2 is exact,
-        // so this equals seeding from A as BBFGPermanentCalculator.hpp:143-167).
+    // contiguous run of groups [g0, g1) for this wave (sizes differ by at most one)
+    const int64_t G = (int64_t)1 << H;
+    const int64_t g0 = (G * wave) / nwaves;
+    const int64_t g1 = (G * (wave + 1)) / nwaves;
+
+    double tr = 0.0, ti = 0.0;  // lane total, lane sign applied at the end
+    if (g0 < g1) {
+        // ---- seed for group g0: delta_0 = +1, walk rows +1, lane row R-1-K-b is
+        // -1 iff bit b of the lane, group row R-1-K-B-h is -1 iff bit h of g0.
+        // Built from the +2A rows and halved (scaling by 2 is exact, so this
+        // equals seeding from A as BBFGPermanentCalculator.hpp:143-167).
         double cr[NC], ci[NC];
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
@@ -143,10 +155,15 @@ __global__ __launch_bounds__(256) void plain_sweep_kernel(PlainParams p) {
         }
         for (int i = R - K; i < R; ++i)
             add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + (size_t)i * rowb));
-        for (int h = 0; h < H; ++h) {
-            const int row = R - 1 - K - h;
-            const double s = ((c >> h) & 1) ? -1.0 : 1.0;
+        for (int b = 0; b < B; ++b) {
+            const int row = R - 1 - K - b;
+            const double s = ((lane >> b) & 1) ? -1.0 : 1.0;
             add_row_signed<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + (size_t)row * rowb), s);
+        }
+        for (int h = 0; h < H; ++h) {
+            const int row = R - 1 - K - B - h;
+            const unsigned off = (((g0 >> h) & 1) ? neg : 0u) + (unsigned)row * rowb;
+            add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off));
         }
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
@@ -154,29 +171,47 @@ __global__ __launch_bounds__(256) void plain_sweep_kernel(PlainParams p) {
             ci[j] *= 0.5;
         }
 
-        double pr = 0.0, pi = 0.0, mr = 0.0, mi = 0.0;
-        // t = 0: the seed itself (even Gray index)
-        add_term<NC>(cr, ci, pr, pi);
-        if (K >= 1) {
-            // t = 1: bit 0 (row R-1) goes to delta = -1
-            add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + low_m));
-            add_term<NC>(cr, ci, mr, mi);
-            for (unsigned t = 2; t < W; t += 2) {
-                const unsigned b = __builtin_ctz(t);
-                const unsigned nb = ((t >> b) ^ (t >> (b + 1))) & 1u;
-                const unsigned off = (nb ? neg : 0u) + (unsigned)(R - 1 - (int)b) * rowb;
-                add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off));
-                add_term<NC>(cr, ci, pr, pi);
-                const unsigned off0 = ((t >> 1) & 1u) ? low_p : low_m;
-                add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off0));
+        for (int64_t g = g0;;) {
+            double pr = 0.0, pi = 0.0, mr = 0.0, mi = 0.0;
+            // t = 0: the group's first term (even Gray index)
+            add_term<NC>(cr, ci, pr, pi);
+            if (K >= 1) {
+                // t = 1: walk bit 0 (row R-1) goes to delta = -1
+                add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + low_m));
                 add_term<NC>(cr, ci, mr, mi);
+                for (unsigned t = 2; t < W; t += 2) {
+                    const unsigned b = __builtin_ctz(t);
+                    const unsigned nb = ((t >> b) ^ (t >> (b + 1))) & 1u;
+                    const unsigned off = (nb ? neg : 0u) + (unsigned)(R - 1 - (int)b) * rowb;
+                    add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off));
+                    add_term<NC>(cr, ci, pr, pi);
+                    const unsigned off0 = ((t >> 1) & 1u) ? low_p : low_m;
+                    add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off0));
+                    add_term<NC>(cr, ci, mr, mi);
+                }
+            }
+            // sign of the group's first term relative to the lane: parity of g
+            const double gs = (__popcll((unsigned long long)g) & 1) ? -1.0 : 1.0;
+            tr = fma(gs, pr - mr, tr);
+            ti = fma(gs, pi - mi, ti);
+            if (++g >= g1) break;
+            // to the next group's seed: the walk ended at Gray code 10..0 (row
+            // R-K at delta = -1), restore it; then flip the group rows whose
+            // bits differ between g-1 and g (wave-uniform).
+            if (K >= 1) add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + (size_t)(R - K) * rowb));
+            const int64_t diff = (g - 1) ^ g;
+            for (int h = 0; h < H; ++h) {
+                if ((diff >> h) & 1) {
+                    const int row = R - 1 - K - B - h;
+                    const unsigned off = (((g >> h) & 1) ? neg : 0u) + (unsigned)row * rowb;
+                    add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off));
+                }
             }
         }
-        // sign of the chunk's first term: parity of its high bits
-        const double cs = (__popcll((unsigned long long)c) & 1) ? -1.0 : 1.0;
-        tr = fma(cs, pr - mr, tr);
-        ti = fma(cs, pi - mi, ti);
     }
+    const double ls = (__popc((unsigned)lane) & 1) ? -1.0 : 1.0;
+    tr = lane_on ? ls * tr : 0.0;
+    ti = lane_on ? ls * ti : 0.0;
 
     ddc v = block_reduce(ddc_from(tr, ti), red);
 

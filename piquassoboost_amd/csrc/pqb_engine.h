@@ -16,15 +16,15 @@ constexpr size_t kCounterBytes = 256;  // grid-reduction counter slot at workspa
 constexpr int kMaxBlocks = 4096;       // partials capacity of the device-API workspace
 
 struct PlainPlan {
-    int K = 0;            // Gray bits walked inside a chunk
-    int H = 0;            // chunk-index bits
-    int64_t nchunks = 1;  // 2^H
-    int64_t lanes = 64;   // launched lanes
+    int K = 0;            // walk bits
+    int B = 0;            // lane bits
+    int H = 0;            // group bits
+    int64_t waves = 1;    // launched waves
     int block = 64;       // threads per block
     int blocks = 1;
 };
 
-PlainPlan make_plain_plan(int R, int NC, int64_t lanes_resident);
+PlainPlan make_plain_plan(int R, int NC, int64_t waves_resident);
 
 // One planned repeated-row permanent (see batch.hip).
 struct TaskPlan {
@@ -61,7 +61,7 @@ struct Engine {
 
     int ensure_init_locked(int dev);
     int ensure_buffers_locked(size_t dneed, size_t hneed);
-    int64_t resident_lanes_locked(int nc);
+    int64_t resident_waves_locked(int nc);
     void free_buffers();
 };
 

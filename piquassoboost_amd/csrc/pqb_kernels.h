@@ -14,8 +14,8 @@ struct PlainParams {
     const double2* M2;   // 2R x NC, packed: rows [0,R) = +2A, rows [R,2R) = -2A
     int R;               // rows
     int K;               // Gray bits walked inside a chunk (rows R-K .. R-1)
-    int H;               // chunk-index bits (rows 1 .. R-1-K), H = R-1-K
-    int64_t nchunks;     // 2^H; lane g takes chunks g, g+L, ... (L = launched lanes)
+    int B;               // lane bits (rows R-K-B .. R-K-1), B <= 6
+    int H;               // group bits (rows 1 .. R-1-K-B), K + B + H = R - 1
     ddc* partials;       // one dd complex per block
     unsigned* counter;   // zero on entry; the last block restores it to zero
     double* out;         // [re, im, hi_re, lo_re, hi_im, lo_im]

@@ -122,16 +122,20 @@ def test_state_length_mismatch_raises():
 
 
 def test_plan_is_host_only():
-    p = C.plain_plan(30, 30, lanes_resident=131072)
-    assert p["K"] + p["H"] == 29
-    assert p["lanes"] <= 131072 and p["lanes"] % 64 == 0
-    assert p["blocks"] * p["block"] == p["lanes"]
-    small = C.plain_plan(3, 3, lanes_resident=131072)
-    assert small["K"] + small["H"] == 2 and small["lanes"] == 64
+    p = C.plain_plan(30, 30, waves_resident=3072)
+    assert p["K"] + p["B"] + p["H"] == 29
+    assert p["waves"] == 3072 and p["blocks"] * p["block"] == 64 * p["waves"]
+    # about 32+ groups per wave, runs differ by at most one group
+    assert 2 ** p["H"] >= 32 * p["waves"]
+    small = C.plain_plan(3, 3, waves_resident=3072)
+    assert small["K"] + small["B"] + small["H"] == 2 and small["waves"] == 1
     for n in range(1, 41):
-        q = C.plain_plan(n, n, lanes_resident=196608)
-        assert q["K"] + q["H"] == n - 1
-        assert 0 <= q["K"] <= 14
+        for wr in (1, 7, 1024, 3072, 4096):
+            q = C.plain_plan(n, n, waves_resident=wr)
+            assert q["K"] + q["B"] + q["H"] == n - 1
+            assert 0 <= q["K"] <= 12 and 0 <= q["B"] <= 6
+            assert 1 <= q["waves"] <= max(wr, 1) and q["waves"] <= 2 ** q["H"]
+            assert q["blocks"] * q["block"] == 64 * q["waves"]
 
 
 def test_exact_route_bounds():
