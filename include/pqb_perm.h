@@ -153,6 +153,12 @@ int pqb_perm_batch(const pqb_task* tasks, int64_t ntasks, int algo, int precisio
 int pqb_perm_batch_tuned(const pqb_task* tasks, int64_t ntasks, int algo, int precision,
                          int64_t walk_length, double* out);
 
+/* Walk lengths (Gray steps per chunk) chosen by the autotuner, indexed by the
+ * task's photon count: walk[n] for n < count, 0 = built-in choice; NULL resets
+ * all.  Applies to later pqb_perm_batch / pqb_perm_repeated calls.  No
+ * reference counterpart. */
+int pqb_set_walk_lengths(const int64_t* walk, int64_t count);
+
 /* Replays the graph of the last batch `iters` times on the engine stream with
  * its inputs already resident and reports the elapsed milliseconds between
  * HIP events around the replays.  No reference counterpart (measurement). */

@@ -52,6 +52,7 @@ SIGNATURES = {
     "pqb_perm_batch_tuned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int64, _dp]),
     "pqb_batch_replay": (ctypes.c_int, [ctypes.c_int, _dp]),
+    "pqb_set_walk_lengths": (ctypes.c_int, [_i64p, ctypes.c_int64]),
     "pqb_repeated_plan": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
                                          ctypes.c_int64, _i64p]),
     "pqb_perm_device_workspace": (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
@@ -91,7 +92,17 @@ def lib():
                     fn.restype = res
                     fn.argtypes = args
                 _lib = L
+                _apply_saved_tuning()
     return _lib
+
+
+def _apply_saved_tuning() -> None:
+    """Walk lengths found by the autotuner (piquassoboost_amd/tuning/*.json)."""
+    from . import autotune
+
+    table = autotune.load()
+    if table:
+        set_walk_lengths(table)
 
 
 def _check(rc: int) -> None:
@@ -193,6 +204,18 @@ def permanent_batch(tasks, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE, 
         _check(lib().pqb_perm_batch(ctypes.cast(arr, ctypes.c_void_p), len(tasks), algo, precision, out))
     del keep
     return [complex(out[2 * i], out[2 * i + 1]) for i in range(len(tasks))]
+
+
+def set_walk_lengths(table) -> None:
+    """Walk length per photon count (dict n -> steps, or None to reset)."""
+    if not table:
+        _check(lib().pqb_set_walk_lengths(None, 0))
+        return
+    arr = np.zeros(MAX_COLS + 1, dtype=np.int64)
+    for n, L in table.items():
+        if 0 <= int(n) <= MAX_COLS:
+            arr[int(n)] = int(L)
+    _check(lib().pqb_set_walk_lengths(arr.ctypes.data_as(_i64p), arr.shape[0]))
 
 
 def batch_replay_ms(iters: int) -> float:

@@ -325,12 +325,16 @@ static int batch_graph_locked(int range, hipGraphExec_t* out) {
     return PQB_OK;
 }
 
+// Walk lengths set by the autotuner, indexed by photon count (0 = automatic).
+static int64_t g_walk[PQB_MAX_COLS + 1] = {0};
+
 // Target walk length for a task of S steps when the batch holds `total`
 // steps: long walks amortise the per-chunk seed, but there must be enough
 // chunks to keep every resident lane busy.
 static uint64_t choose_L(long double S, long double total, int64_t lanes) {
     long double want_chunks = std::max<long double>(4.0L * (long double)lanes * S / std::max(total, S), 1.0L);
     long double L = S / want_chunks;
+    L = std::min(L, S / 64.0L);  // at least one chunk per lane of a wave
     L = std::min<long double>(std::max<long double>(L, 16.0L), 2048.0L);
     return (uint64_t)L;
 }
@@ -564,7 +568,12 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     const int64_t lanes = (int64_t)batch_grid_locked(2) * 256;
     for (int64_t i = 0; i < ntasks; ++i) {
         if (plans[(size_t)i].trivial) continue;
-        const uint64_t L = L_override > 0 ? (uint64_t)L_override : choose_L(steps[(size_t)i], total, lanes);
+        int64_t photons = 0;
+        for (int64_t j = 0; j < tasks[i].cols; ++j) photons += tasks[i].col_mult[j];
+        const int64_t tuned = (photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons] : 0;
+        const uint64_t L = L_override > 0 ? (uint64_t)L_override
+                           : tuned > 0    ? (uint64_t)tuned
+                                          : choose_L(steps[(size_t)i], total, lanes);
         st = plan_task(tasks[i].A, tasks[i].rows, tasks[i].cols, tasks[i].ld, tasks[i].row_mult, tasks[i].col_mult,
                        L, plans[(size_t)i]);
         if (st) return st;
@@ -620,6 +629,12 @@ int pqb_perm_batch_tuned(const pqb_task* tasks, int64_t ntasks, int algo, int pr
     } catch (const std::exception& ex) {
         return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
     }
+}
+
+int pqb_set_walk_lengths(const int64_t* walk, int64_t count) {
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    for (int i = 0; i <= PQB_MAX_COLS; ++i) g_walk[i] = (walk && i < count && walk[i] > 0) ? walk[i] : 0;
+    return PQB_OK;
 }
 
 int pqb_batch_replay(int iters, double* ms) {

@@ -21,6 +21,35 @@ def haar_unitary(m: int, seed: int) -> np.ndarray:
     return np.ascontiguousarray(Q * (d / np.abs(d))[None, :])
 
 
+def uneven_batch(seed: int = 1005, count: int = 4096, modes: int = 100, n_min: int = 8, n_max: int = 26,
+                 mult_p=(0.7, 0.2, 0.1)):
+    """Batch of boson-sampling transitions of uneven size on one Haar interferometer.
+
+    Per task: n ~ U{n_min..n_max} photons in n distinct input modes (one each);
+    output multiplicities drawn iid from {1, 2, 3} with probabilities mult_p,
+    the last one truncated so that they sum to n, placed on distinct output
+    modes.  Returns (U, [(input_state, output_state), ...]) over `modes` modes.
+    """
+    U = haar_unitary(modes, seed)
+    rng = np.random.default_rng(seed + 1)
+    tasks = []
+    for _ in range(count):
+        n = int(rng.integers(n_min, n_max + 1))
+        inp = np.zeros(modes, dtype=np.int64)
+        inp[rng.choice(modes, size=n, replace=False)] = 1
+        mults = []
+        left = n
+        while left > 0:
+            m = int(rng.choice([1, 2, 3], p=mult_p))
+            m = min(m, left)
+            mults.append(m)
+            left -= m
+        outp = np.zeros(modes, dtype=np.int64)
+        outp[rng.choice(modes, size=len(mults), replace=False)] = mults
+        tasks.append((inp, outp))
+    return U, tasks
+
+
 def adapt_interferometer(U, input_state, output_state) -> np.ndarray:
     """Rows of occupied output modes x columns of occupied input modes."""
     U = np.asarray(U)
