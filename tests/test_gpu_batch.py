@@ -42,18 +42,20 @@ def test_c4_against_oracle(c4):
 def test_c4_full_mode_form_matches_compact(c4):
     """The same transitions given on the full 100-mode unitary with zero
     multiplicities (what BBFGRepeatedPermanentCalculator receives from a
-    sampler) give the same values."""
+    sampler) give the same values.  The walk length depends on the batch's
+    total work, so a 256-task batch is chunked differently from the
+    4096-task one: agreement is to the parity bar, not bit for bit."""
     U, pairs, tasks, res = c4
     full = C.permanent_batch([(U, outp, inp) for inp, outp in pairs[:256]])
     for k in range(256):
-        assert rel_err(full[k], res[k]) < 1e-12
+        assert rel_err(full[k], res[k]) < 1e-10
 
 
 def test_c4_per_call_equals_batch(c4):
     U, pairs, tasks, res = c4
     for k in range(0, len(tasks), 97):
         A, rows, cols = tasks[k]
-        assert rel_err(C.permanent_repeated(A, rows, cols), res[k]) < 1e-12
+        assert rel_err(C.permanent_repeated(A, rows, cols), res[k]) < 1e-10
 
 
 def test_c4_replay_deterministic(c4):
