@@ -193,6 +193,38 @@ int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
  * resident for this kernel (0 = ask the device). */
 int pqb_plain_plan(int64_t rows, int64_t cols, int64_t waves_resident, int64_t* plan);
 
+/* ---- permanent with its gradient (adjoint sweep) ---------------------------
+ * For a square n x n complex matrix: perm_out = perm(A) (2 doubles) and
+ * grad_out[i*n + j] = d perm(A) / d a_ij = perm(A without row i and column j)
+ * (n*n complex, packed row-major), from one Gray-code sweep.  The reference
+ * has no derivative; the same numbers come from n*n calls of
+ * BBFGPermanentCalculator::calculate on the (n-1) x (n-1) minors
+ * (BBFGPermanentCalculator.h:66, .cpp:88-126), which the tests compare with.
+ * FP64 sweep, double-double reduction.  n <= PQB_MAX_COLS; n = 0 gives
+ * perm 1 and no gradient.  Host pointers, synchronous. */
+int pqb_perm_grad(const double* A, int64_t n, int64_t ld, double* perm_out, double* grad_out);
+
+/* Batch of `count` equal-size matrices: matrix b, element (i, j) at
+ * A[2 * (b * stride + i * ld + j)]; perm_out: count complex; grad_out:
+ * count x n x n complex, packed.  One launch for the whole batch. */
+int pqb_perm_grad_batch(const double* A, int64_t count, int64_t n, int64_t ld, int64_t stride,
+                        double* perm_out, double* grad_out);
+
+/* Device-resident form (inputs already in HBM, e.g. a PyTorch tensor):
+ * dA packed count x n x n complex128; d_perm count complex; d_grad
+ * count x n x n complex; d_work at least pqb_perm_grad_workspace bytes (no
+ * initial contents needed).  Asynchronous on `stream`; allocates nothing.
+ * No reference counterpart (the form a torch.autograd.Function calls). */
+int pqb_perm_grad_workspace(int64_t count, int64_t n, size_t* bytes);
+int pqb_perm_grad_device(const double* dA, int64_t count, int64_t n, double* d_perm, double* d_grad,
+                         void* d_work, size_t work_bytes, void* stream);
+
+/* Launch plan of the adjoint sweep: plan[0] = M (columns per lane), [1] = K
+ * (register bits), [2] = L (lane bits), [3] = H (high bits), [4] = waves per
+ * matrix, [5] = blocks, [6] = LDS bytes per block.  waves_resident > 0: plan
+ * for that many resident waves without a device; 0: ask the device. */
+int pqb_grad_plan(int64_t count, int64_t n, int64_t waves_resident, int64_t* plan);
+
 /* ---- timing helper ---------------------------------------------------------
  * Runs `iters` back-to-back pqb_perm_device launches on the engine's stream
  * and reports the elapsed time between HIP events recorded on that stream

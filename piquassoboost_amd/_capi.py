@@ -61,6 +61,13 @@ SIGNATURES = {
     "pqb_plain_plan": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p]),
     "pqb_time_perm_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_int, _dp]),
+    "pqb_perm_grad": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, _dp, _dp]),
+    "pqb_perm_grad_batch": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                           _dp, _dp]),
+    "pqb_perm_grad_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    "pqb_perm_grad_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "pqb_grad_plan": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p]),
 }
 
 _lib = None
@@ -268,3 +275,49 @@ def time_permanent_device(dA_ptr: int, rows: int, cols: int, d_out_ptr: int, ite
     ms = ctypes.c_double(0.0)
     _check(lib().pqb_time_perm_device(dA_ptr, rows, cols, d_out_ptr, iters, ctypes.byref(ms)))
     return ms.value
+
+
+# ---- permanent with its gradient (adjoint sweep) ----
+
+def permanent_grad(A) -> tuple[complex, np.ndarray]:
+    """(perm(A), G) with G[i, j] = d perm / d a_ij = perm of the (i, j) minor; A square."""
+    M = as_matrix(A)
+    n = M.shape[0]
+    if M.shape[1] != n:
+        raise EngineError(PQB_ERR_ARG, "the gradient needs a square matrix")
+    out = (ctypes.c_double * 2)()
+    G = np.zeros((n, n), dtype=np.complex128)
+    _check(lib().pqb_perm_grad(M.ctypes.data_as(_dp), n, n, out, G.ctypes.data_as(_dp)))
+    return complex(out[0], out[1]), G
+
+
+def permanent_grad_batch(As) -> tuple[np.ndarray, np.ndarray]:
+    """Batch of equal-size square matrices (count x n x n) -> (perms[count], grads[count, n, n])."""
+    B = np.ascontiguousarray(As, dtype=np.complex128)
+    if B.ndim != 3 or B.shape[1] != B.shape[2]:
+        raise EngineError(PQB_ERR_ARG, "expected a count x n x n batch of square matrices")
+    count, n = B.shape[0], B.shape[1]
+    perms = np.zeros(count, dtype=np.complex128)
+    G = np.zeros((count, n, n), dtype=np.complex128)
+    _check(lib().pqb_perm_grad_batch(B.ctypes.data_as(_dp), count, n, n, n * n, perms.ctypes.data_as(_dp),
+                                     G.ctypes.data_as(_dp)))
+    return perms, G
+
+
+def grad_workspace_bytes(count: int, n: int) -> int:
+    b = ctypes.c_size_t(0)
+    _check(lib().pqb_perm_grad_workspace(count, n, ctypes.byref(b)))
+    return int(b.value)
+
+
+def permanent_grad_device(dA_ptr: int, count: int, n: int, d_perm_ptr: int, d_grad_ptr: int, d_work_ptr: int,
+                          work_bytes: int, stream: int = 0) -> None:
+    """Asynchronous adjoint sweep on device pointers (see pqb_perm_grad_device)."""
+    _check(lib().pqb_perm_grad_device(dA_ptr, count, n, d_perm_ptr, d_grad_ptr, d_work_ptr, work_bytes,
+                                      stream or None))
+
+
+def grad_plan(count: int, n: int, waves_resident: int = 0) -> dict:
+    p = (ctypes.c_int64 * 7)()
+    _check(lib().pqb_grad_plan(count, n, waves_resident, p))
+    return {"M": p[0], "K": p[1], "L": p[2], "H": p[3], "wpt": p[4], "blocks": p[5], "lds": p[6]}

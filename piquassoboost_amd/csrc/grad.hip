@@ -1,0 +1,259 @@
+// Host side of the adjoint sweep (perm_grad.hip): launch plan, workspace
+// layout and the C-ABI entry points for the permanent together with its
+// gradient G_ij = d perm(A) / d a_ij of square complex matrices, one matrix or
+// a batch of equal-size matrices per call.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "pqb_perm.h"
+#include "pqb_device.h"
+#include "pqb_kernels.h"
+#include "pqb_engine.h"
+
+namespace pqb {
+
+struct GradPlan {
+    int M = 1;            // columns per lane (4 lanes per sign vector)
+    int K = 0;            // register bits (walked inside a block)
+    int L = 0;            // delta-lane bits
+    int H = 0;            // high bits
+    int64_t wpt = 4;      // waves per task
+    int64_t blocks = 1;   // launched blocks (4 waves each)
+    size_t lds = 0;       // dynamic LDS bytes per block
+};
+
+// Register bits per column count: the largest K whose accumulators and row
+// data (4 M doubles each) still fit next to the column sums, running sums and
+// suffix products in 256 VGPRs (the instantiations of perm_grad.hip).
+static int grad_register_bits(int M) { return M <= 5 ? 3 : (M <= 7 ? 2 : 1); }
+
+static size_t grad_lds_bytes(int n, int M, int H) {
+    const size_t NP = 4 * (size_t)M;
+    return sizeof(double2) * ((size_t)n * NP + 4 * (16 + (size_t)H) * NP);
+}
+
+static GradPlan grad_shape(int n) {
+    GradPlan g;
+    g.M = (n + 3) / 4;
+    g.L = std::min(4, n - 1);
+    g.K = std::min(grad_register_bits(g.M), n - 1 - g.L);
+    g.H = n - 1 - g.L - g.K;
+    g.lds = grad_lds_bytes(n, g.M, g.H);
+    return g;
+}
+
+// Waves per task: about four rounds of the resident waves over the batch,
+// at least 4 (one block), at most the number of blocks of 2^K steps.
+static GradPlan make_grad_plan(int n, int64_t count, int64_t waves_resident) {
+    GradPlan g = grad_shape(n);
+    const int64_t NB = (int64_t)1 << g.H;
+    const int64_t target = std::max<int64_t>(4 * std::max<int64_t>(waves_resident, 1), 4);
+    int64_t per = (target + count - 1) / count;
+    per = ((per + 3) / 4) * 4;
+    const int64_t cap = std::max<int64_t>(4, ((NB + 3) / 4) * 4);
+    g.wpt = std::max<int64_t>(4, std::min(per, cap));
+    g.blocks = count * g.wpt / 4;
+    return g;
+}
+
+static size_t grad_partials_bytes(int n, int64_t count, int64_t wpt) {
+    return (size_t)count * (size_t)wpt * ((size_t)n * n + 1) * sizeof(double2);
+}
+
+// Resident waves of the adjoint kernel for this shape (engine lock held).
+static int grad_resident_waves_locked(const GradPlan& g, int64_t* waves) {
+    const void* fn = grad_kernel_fn(g.M, g.K);
+    if (!fn) return fail(PQB_ERR_SIZE, "no adjoint kernel for this size");
+    static bool attr_set[11][4] = {};
+    if (!attr_set[g.M][g.K]) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(adjoint kernel)");
+        attr_set[g.M][g.K] = true;
+    }
+    int nb = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, g.lds);
+    if (e != hipSuccess) return hip_fail(e, "occupancy query (adjoint kernel)");
+    if (nb <= 0) return fail(PQB_ERR_SIZE, "adjoint kernel does not fit on a compute unit");
+    *waves = (int64_t)nb * 4 * g_eng.cus;
+    return PQB_OK;
+}
+
+static int grad_plan_locked(int n, int64_t count, GradPlan* out) {
+    int st = g_eng.ensure_init_locked(-1);
+    if (st) return st;
+    const GradPlan shape = grad_shape(n);
+    int64_t wr = 0;
+    st = grad_resident_waves_locked(shape, &wr);
+    if (st) return st;
+    *out = make_grad_plan(n, count, wr);
+    return PQB_OK;
+}
+
+static int grad_check(int64_t count, int64_t n) {
+    if (count < 1) return fail(PQB_ERR_ARG, "batch count must be positive");
+    if (n < 1) return fail(PQB_ERR_ARG, "matrix order must be positive");
+    if (n > PQB_MAX_COLS)
+        return fail(PQB_ERR_SIZE, "matrix order above " + std::to_string(PQB_MAX_COLS));
+    return PQB_OK;
+}
+
+// Launch on `st`: dA packed count x n x n, outputs packed.  `work` holds the
+// partials (grad_partials_bytes).
+static int grad_launch(const GradPlan& g, const double2* dA, int n, int64_t count, double2* d_perm,
+                       double2* d_grad, void* work, hipStream_t st) {
+    if (g.blocks > 0x7fffffff) return fail(PQB_ERR_SIZE, "adjoint launch too large");
+    GradParams p;
+    p.A = dA;
+    p.partials = static_cast<double2*>(work);
+    p.n = n;
+    p.L = g.L;
+    p.H = g.H;
+    p.wpt = (int)g.wpt;
+    hipError_t e = launch_grad(g.M, g.K, p, (int)g.blocks, g.lds, st);
+    if (e != hipSuccess) return hip_fail(e, "adjoint sweep launch");
+    GradReduceParams r;
+    r.partials = p.partials;
+    r.perm = d_perm;
+    r.grad = d_grad;
+    r.n = n;
+    r.wpt = (int)g.wpt;
+    r.count = (int)count;
+    r.scale = std::ldexp(1.0, -(n - 1));
+    e = launch_grad_reduce(r, st);
+    if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
+    return PQB_OK;
+}
+
+static int grad_host_entry(const double* A, int64_t count, int64_t n, int64_t ld, int64_t stride,
+                           double* perm_out, double* grad_out) {
+    if (!perm_out) return fail(PQB_ERR_ARG, "output pointer is NULL");
+    if (count < 1) return fail(PQB_ERR_ARG, "batch count must be positive");
+    if (n == 0) {  // perm of the empty matrix is 1; the gradient is empty
+        for (int64_t b = 0; b < count; ++b) {
+            perm_out[2 * b] = 1.0;
+            perm_out[2 * b + 1] = 0.0;
+        }
+        return PQB_OK;
+    }
+    int st = grad_check(count, n);
+    if (st) return st;
+    if (!A || !grad_out) return fail(PQB_ERR_ARG, "matrix or gradient pointer is NULL");
+    if (ld < n) return fail(PQB_ERR_ARG, "leading dimension smaller than the matrix order");
+    if (count > 1 && stride < n * ld) return fail(PQB_ERR_ARG, "matrix stride smaller than n * ld");
+    const int nn = (int)n;
+    const size_t mat = (size_t)nn * nn;
+
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    GradPlan g;
+    st = grad_plan_locked(nn, count, &g);
+    if (st) return st;
+    const size_t a_bytes = (size_t)count * mat * sizeof(double2);
+    const size_t out_bytes = (size_t)count * (mat + 1) * sizeof(double2);
+    const size_t part_bytes = grad_partials_bytes(nn, count, g.wpt);
+    const size_t a_off = kCounterBytes;  // offset 0 holds the plain sweep's grid counter
+    const size_t out_off = a_off + ((a_bytes + 255) / 256) * 256;
+    const size_t part_off = out_off + ((out_bytes + 255) / 256) * 256;
+    st = g_eng.ensure_buffers_locked(part_off + part_bytes, std::max(a_bytes, out_bytes));
+    if (st) return st;
+    double* h = static_cast<double*>(g_eng.hbuf);
+    for (int64_t b = 0; b < count; ++b)
+        for (int i = 0; i < nn; ++i)
+            std::memcpy(h + 2 * ((size_t)b * mat + (size_t)i * nn), A + 2 * (b * stride + i * ld),
+                        (size_t)nn * 2 * sizeof(double));
+    char* d = static_cast<char*>(g_eng.dbuf);
+    hipError_t e = hipMemcpyAsync(d + a_off, h, a_bytes, hipMemcpyHostToDevice, g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+    double2* dperm = reinterpret_cast<double2*>(d + out_off);
+    double2* dgrad = dperm + count;
+    st = grad_launch(g, reinterpret_cast<const double2*>(d + a_off), nn, count, dperm, dgrad, d + part_off,
+                     g_eng.stream);
+    if (st) return st;
+    // the staging buffer is reused for the results once the upload has been consumed
+    e = hipMemcpyAsync(h, d + out_off, out_bytes, hipMemcpyDeviceToHost, g_eng.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "adjoint sweep");
+    std::memcpy(perm_out, h, (size_t)count * 2 * sizeof(double));
+    std::memcpy(grad_out, h + 2 * count, (size_t)count * mat * 2 * sizeof(double));
+    return PQB_OK;
+}
+
+}  // namespace pqb
+
+using namespace pqb;
+
+extern "C" {
+
+int pqb_perm_grad(const double* A, int64_t n, int64_t ld, double* perm_out, double* grad_out) {
+    try {
+        return grad_host_entry(A, 1, n, ld, n * ld, perm_out, grad_out);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+int pqb_perm_grad_batch(const double* A, int64_t count, int64_t n, int64_t ld, int64_t stride,
+                        double* perm_out, double* grad_out) {
+    try {
+        return grad_host_entry(A, count, n, ld, stride, perm_out, grad_out);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+int pqb_perm_grad_workspace(int64_t count, int64_t n, size_t* bytes) {
+    if (!bytes) return fail(PQB_ERR_ARG, "bytes pointer is NULL");
+    int st = grad_check(count, n);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    GradPlan g;
+    st = grad_plan_locked((int)n, count, &g);
+    if (st) return st;
+    *bytes = grad_partials_bytes((int)n, count, g.wpt);
+    return PQB_OK;
+}
+
+int pqb_perm_grad_device(const double* dA, int64_t count, int64_t n, double* d_perm, double* d_grad,
+                         void* d_work, size_t work_bytes, void* stream) {
+    if (!dA || !d_perm || !d_grad || !d_work) return fail(PQB_ERR_ARG, "NULL device pointer");
+    int st = grad_check(count, n);
+    if (st) return st;
+    GradPlan g;
+    {
+        std::lock_guard<std::mutex> lk(g_eng.mu);
+        st = grad_plan_locked((int)n, count, &g);
+        if (st) return st;
+    }
+    if (work_bytes < grad_partials_bytes((int)n, count, g.wpt)) return fail(PQB_ERR_ARG, "workspace too small");
+    return grad_launch(g, reinterpret_cast<const double2*>(dA), (int)n, count, reinterpret_cast<double2*>(d_perm),
+                       reinterpret_cast<double2*>(d_grad), d_work, static_cast<hipStream_t>(stream));
+}
+
+int pqb_grad_plan(int64_t count, int64_t n, int64_t waves_resident, int64_t* plan) {
+    if (!plan) return fail(PQB_ERR_ARG, "plan pointer is NULL");
+    int st = grad_check(count, n);
+    if (st) return st;
+    GradPlan g;
+    if (waves_resident > 0) {
+        g = make_grad_plan((int)n, count, waves_resident);
+    } else {
+        std::lock_guard<std::mutex> lk(g_eng.mu);
+        st = grad_plan_locked((int)n, count, &g);
+        if (st) return st;
+    }
+    plan[0] = g.M;
+    plan[1] = g.K;
+    plan[2] = g.L;
+    plan[3] = g.H;
+    plan[4] = g.wpt;
+    plan[5] = g.blocks;
+    plan[6] = (int64_t)g.lds;
+    return PQB_OK;
+}
+
+}  // extern "C"

@@ -88,6 +88,28 @@ int rep_range_for(int min_ns, int max_ns);
 const void* rep_batch_kernel_fn(int range);
 hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks, hipStream_t st);
 
+// ---- adjoint sweep (permanent + all first derivatives of square matrices) ----
+struct GradParams {
+    const double2* A;    // count x n x n, packed row-major
+    double2* partials;   // count x wpt x (n*n + 1): G row-major, then the permanent
+    int n;               // rows = columns
+    int L;               // delta-lane bits (<= 4)
+    int H;               // high bits: K + L + H = n - 1 (K is the kernel's template argument)
+    int wpt;             // waves per task, multiple of 4
+};
+
+struct GradReduceParams {
+    const double2* partials;
+    double2* perm;       // count
+    double2* grad;       // count x n x n
+    int n, wpt, count;
+    double scale;        // 2^-(n-1)
+};
+
+const void* grad_kernel_fn(int M, int K);
+hipError_t launch_grad(int M, int K, const GradParams& p, int nblocks, size_t lds_bytes, hipStream_t st);
+hipError_t launch_grad_reduce(const GradReduceParams& p, hipStream_t st);
+
 hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, hipStream_t st);
 hipError_t launch_plain_prep(const double2* A, double2* M2, int n, hipStream_t st);
 hipError_t launch_plain_exact(int nc, const ExactParams& p, int nblocks, int bdim, hipStream_t st);

@@ -50,6 +50,29 @@ def uneven_batch(seed: int = 1005, count: int = 4096, modes: int = 100, n_min: i
     return U, tasks
 
 
+def gradient_blocks(seed: int = 1006, modes: int = 50, sizes=((20, 256), (26, 8))):
+    """Square blocks of one Haar interferometer for gradient workloads.
+
+    For each (n, count) in `sizes`: `count` n x n submatrices U[rows, cols]
+    with rows and columns drawn as sorted random n-subsets of the `modes`
+    modes (seeded).  Returns (U, {n: array count x n x n, ...},
+    {n: [(rows, cols), ...]})."""
+    U = haar_unitary(modes, seed)
+    rng = np.random.default_rng(seed + 1)
+    blocks, index = {}, {}
+    for n, count in sizes:
+        B = np.empty((count, n, n), dtype=np.complex128)
+        idx = []
+        for b in range(count):
+            r = np.sort(rng.choice(modes, size=n, replace=False))
+            c = np.sort(rng.choice(modes, size=n, replace=False))
+            B[b] = U[np.ix_(r, c)]
+            idx.append((r, c))
+        blocks[n] = B
+        index[n] = idx
+    return U, blocks, index
+
+
 def adapt_interferometer(U, input_state, output_state) -> np.ndarray:
     """Rows of occupied output modes x columns of occupied input modes."""
     U = np.asarray(U)

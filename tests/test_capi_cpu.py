@@ -181,3 +181,36 @@ def test_repeated_plan_host_only():
     # 3 x 2 matrix, rows 1,1,1 (S = 1*2*2 = 4), columns 2,1 (S = 1*3 = 3): transposed
     p2 = C.repeated_plan(np.ones((3, 2)), np.array([1, 1, 1]), np.array([2, 1]), 1 << 20)
     assert p2["walk"] * p2["chunks"] == 3 and p2["slots"] == 3 and p2["rows"] == 2
+
+
+GRAD_INSTANTIATIONS = {(1, 0), (2, 0), (2, 1), (2, 2), (2, 3), (3, 3), (4, 3), (5, 3), (6, 2), (7, 2), (8, 1),
+                       (9, 1), (10, 1)}
+
+
+def test_grad_plan_host_only():
+    """Adjoint sweep plan: the n-1 sign bits split into register (K), lane (L)
+    and high (H) bits, one compiled kernel per (columns per lane, K), whole
+    blocks of 4 waves per matrix, LDS within the 160 KiB of a CU."""
+    for n in range(1, 41):
+        for count, wr in ((1, 1024), (8, 2048), (256, 2048), (5000, 512)):
+            g = C.grad_plan(count, n, wr)
+            assert g["K"] + g["L"] + g["H"] == n - 1
+            assert g["M"] == (n + 3) // 4 and g["L"] == min(4, n - 1)
+            assert (g["M"], g["K"]) in GRAD_INSTANTIATIONS, (n, g)
+            assert g["wpt"] % 4 == 0 and 4 * g["blocks"] == count * g["wpt"]
+            assert g["wpt"] <= max(4, 2 ** g["H"] + 3)
+            assert g["lds"] == 16 * (n * 4 * g["M"] + 4 * (16 + g["H"]) * 4 * g["M"]) <= 160 * 1024
+    # the gradient workload: 256 x n=20 and 8 x n=26 at 2048 resident waves
+    assert C.grad_plan(256, 20, 2048)["wpt"] == 32
+    assert C.grad_plan(8, 26, 2048)["wpt"] == 1024
+
+
+def test_grad_argument_errors_without_device():
+    p, G = C.permanent_grad(np.zeros((0, 0)))
+    assert p == 1.0 and G.shape == (0, 0)
+    with pytest.raises(C.EngineError):
+        C.permanent_grad(np.ones((3, 4)))
+    with pytest.raises(C.EngineError, match="order above"):
+        C.permanent_grad(np.ones((41, 41)))
+    with pytest.raises(C.EngineError):
+        C.grad_plan(0, 5, 1024)
