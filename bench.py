@@ -105,6 +105,83 @@ def batched_leg() -> dict:
     }
 
 
+def grad_leg(reps: int = 10) -> dict:
+    """Gradient workload: 256 n=20 and 8 n=26 blocks of a seeded 50-mode Haar
+    unitary (seed 1006).  Forward = the batched plain sweep of the 264
+    permanents (graph replay, inputs resident).  Forward+backward = the adjoint
+    sweep, which yields the permanents and every d perm / d a_ij (two launches,
+    one per size, timed with HIP events on the launch stream), and the same
+    through torch.autograd (loss sum log|perm|^2, wall clock)."""
+    import torch
+    from piquassoboost_amd import _capi as C
+    from piquassoboost_amd.autograd import permanent
+    from piquassoboost_amd.sampling.utilities import gradient_blocks
+
+    U, blocks, index = gradient_blocks()
+    dev = torch.device("cuda:0")
+    tasks = []
+    for n, B in sorted(blocks.items()):
+        ones = np.ones(n, np.int64)
+        tasks += [(B[b], ones, ones) for b in range(B.shape[0])]
+    C.permanent_batch(tasks)
+    fwd_ms = C.batch_replay_ms(reps) / reps
+
+    stream = torch.cuda.current_stream(dev)
+    bufs = []
+    for n, B in sorted(blocks.items()):
+        count = B.shape[0]
+        dA = torch.from_numpy(B).to(dev)
+        dp = torch.empty(count, dtype=torch.complex128, device=dev)
+        dG = torch.empty_like(dA)
+        wb = C.grad_workspace_bytes(count, n)
+        work = torch.empty(wb, dtype=torch.uint8, device=dev)
+        bufs.append((n, count, dA, dp, dG, work, wb))
+
+    def sweep():
+        for n, count, dA, dp, dG, work, wb in bufs:
+            C.permanent_grad_device(dA.data_ptr(), count, n, dp.data_ptr(), dG.data_ptr(), work.data_ptr(), wb,
+                                    stream.cuda_stream)
+
+    sweep()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        sweep()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    grad_ms = e0.elapsed_time(e1) / reps
+
+    leaves = [torch.from_numpy(B).to(dev).requires_grad_(True) for n, B in sorted(blocks.items())]
+
+    def step():
+        loss = sum(torch.log(permanent(A).abs() ** 2).sum() for A in leaves)
+        loss.backward()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    autograd_ms = (time.perf_counter() - t0) * 1e3 / reps
+
+    steps = sum(B.shape[0] * 2.0 ** (n - 1) for n, B in blocks.items())
+    flops = sum(B.shape[0] * 2.0 ** (n - 1) * (24 * n - 22) for n, B in blocks.items())
+    return {
+        "workload": "256 n=20 + 8 n=26 complex128 blocks of a 50-mode Haar unitary (seed 1006), "
+                    "loss sum log|perm|^2",
+        "gray_steps": steps,
+        "forward_replay_ms": fwd_ms,
+        "fwd_bwd_adjoint_ms": grad_ms,
+        "ratio_fwd_bwd_over_fwd": grad_ms / fwd_ms,
+        "autograd_step_ms": autograd_ms,
+        "adjoint_flops": flops,
+        "adjoint_tflops": flops / (grad_ms * 1e-3) / 1e12,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +189,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
+    ap.add_argument("--no-grad", action="store_true", help="skip the forward+backward leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
                          "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
@@ -206,6 +284,8 @@ def main() -> int:
     }
     if not a.no_batch:
         line["batched"] = batched_leg()
+    if not a.no_grad:
+        line["grad"] = grad_leg()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A)
     print(json.dumps(line), flush=True)
