@@ -185,6 +185,18 @@ size_t pqb_perm_device_workspace(int64_t rows, int64_t cols);
 int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
                     void* d_work, size_t work_bytes, void* stream);
 
+/* Batch of `count` equal-size plain permanents on device pointers: dA packed
+ * count x rows x cols complex128; d_out 6 doubles per matrix (as above);
+ * d_work at least pqb_perm_batch_device_workspace bytes, its first
+ * 4 * count bytes zero before the first call (each call leaves them zero).
+ * One launch for the batch (the matrices share the grid); asynchronous on
+ * `stream`; allocates nothing.  Replaces per-matrix calls of the plain
+ * calculators over a list (GlynnPermanentCalculator_Wrapper.hpp:204-293) for
+ * callers whose matrices are already on the GPU. */
+int pqb_perm_batch_device_workspace(int64_t count, int64_t rows, int64_t cols, size_t* bytes);
+int pqb_perm_batch_device(const double* dA, int64_t count, int64_t rows, int64_t cols, double* d_out,
+                          void* d_work, size_t work_bytes, void* stream);
+
 /* ---- launch plan introspection (host only, no device needed) --------------
  * Writes the sweep decomposition chosen for a rows x cols plain permanent:
  * plan[0] = K (Gray bits walked per group), plan[1] = B (lane bits),

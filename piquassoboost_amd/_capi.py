@@ -68,6 +68,10 @@ SIGNATURES = {
     "pqb_perm_grad_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "pqb_grad_plan": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p]),
+    "pqb_perm_batch_device_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                       ctypes.POINTER(ctypes.c_size_t)]),
+    "pqb_perm_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
 }
 
 _lib = None
@@ -321,3 +325,17 @@ def grad_plan(count: int, n: int, waves_resident: int = 0) -> dict:
     p = (ctypes.c_int64 * 7)()
     _check(lib().pqb_grad_plan(count, n, waves_resident, p))
     return {"M": p[0], "K": p[1], "L": p[2], "H": p[3], "wpt": p[4], "blocks": p[5], "lds": p[6]}
+
+
+# ---- batch of plain permanents on device pointers ----
+
+def batch_device_workspace_bytes(count: int, rows: int, cols: int) -> int:
+    b = ctypes.c_size_t(0)
+    _check(lib().pqb_perm_batch_device_workspace(count, rows, cols, ctypes.byref(b)))
+    return int(b.value)
+
+
+def permanent_batch_device(dA_ptr: int, count: int, rows: int, cols: int, d_out_ptr: int, d_work_ptr: int,
+                           work_bytes: int, stream: int = 0) -> None:
+    """Asynchronous batch of plain permanents on device pointers (see pqb_perm_batch_device)."""
+    _check(lib().pqb_perm_batch_device(dA_ptr, count, rows, cols, d_out_ptr, d_work_ptr, work_bytes, stream or None))

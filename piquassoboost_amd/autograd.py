@@ -9,13 +9,13 @@ PyTorch's convention for complex inputs gives
 
     grad_A = grad_out * conj(G).
 
-Without a gradient the forward pass is the batched plain sweep.  There is no
-CPU path: a tensor that is not on the GPU is rejected.
+Without a gradient the forward pass is the batched plain sweep on the same
+device pointers.  There is no CPU path: a tensor that is not on the GPU is
+rejected.
 """
 
 from __future__ import annotations
 
-import numpy as np
 import torch
 
 from . import _capi as C
@@ -49,10 +49,14 @@ def _grad_sweep(Ab: torch.Tensor, n: int):
 
 
 def _forward_only(Ab: torch.Tensor, n: int) -> torch.Tensor:
-    host = Ab.detach().cpu().numpy()
-    ones = np.ones(n, dtype=np.int64)
-    vals = C.permanent_batch([(host[b], ones, ones) for b in range(host.shape[0])])
-    return torch.tensor(np.asarray(vals, dtype=np.complex128), device=Ab.device)
+    """Batched plain sweep on device pointers (pqb_perm_batch_device)."""
+    count = Ab.shape[0]
+    out = torch.empty((count, 6), dtype=torch.float64, device=Ab.device)
+    wb = C.batch_device_workspace_bytes(count, n, n)
+    work = torch.zeros(wb, dtype=torch.uint8, device=Ab.device)  # per-matrix counters start at zero
+    stream = torch.cuda.current_stream(Ab.device).cuda_stream
+    C.permanent_batch_device(Ab.data_ptr(), count, n, n, out.data_ptr(), work.data_ptr(), wb, stream)
+    return torch.complex(out[:, 0], out[:, 1])
 
 
 class Permanent(torch.autograd.Function):

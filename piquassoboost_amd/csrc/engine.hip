@@ -298,6 +298,7 @@ static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* o
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     PlainParams p;
     p.M2 = reinterpret_cast<const double2*>(d + m2_off);
+    p.m2_stride = (int64_t)m2_elems;
     p.R = R;
     p.K = pl.K;
     p.B = pl.B;
@@ -306,7 +307,7 @@ static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* o
     p.counter = reinterpret_cast<unsigned*>(d);
     p.out = reinterpret_cast<double*>(d + out_off);
     p.scale = std::ldexp(1.0, -(R - 1));
-    e = launch_plain(C, p, pl.blocks, pl.block, g_eng.stream);
+    e = launch_plain(C, p, pl.blocks, pl.block, 1, g_eng.stream);
     if (e != hipSuccess) return hip_fail(e, "plain sweep launch");
     double* hout = h + 2 * m2_elems;
     e = hipMemcpyAsync(hout, d + out_off, 6 * sizeof(double), hipMemcpyDeviceToHost, g_eng.stream);
@@ -382,6 +383,85 @@ int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int alg
     return run_plain_locked(A, R, C, ld, out);
 }
 
+// ----------------------------------------------------- device-pointer API --
+//
+// Workspace of the device entry points: one counter per matrix (rounded up
+// to 256 bytes), then `blocks_per_matrix` dd partials per matrix, then the
+// +-2A tables of all matrices.  For one matrix the layout is the documented
+// pqb_perm_device one (counter slot of 256 bytes, kMaxBlocks partials).
+
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t plain_device_bytes(int64_t count, int64_t rows, int64_t cols, int64_t blocks_per_matrix) {
+    return round256((size_t)count * sizeof(unsigned)) + (size_t)count * blocks_per_matrix * sizeof(ddc) +
+           (size_t)count * 2 * rows * cols * sizeof(double2);
+}
+
+int plain_device_check(int64_t count, int64_t rows, int64_t cols) {
+    if (count < 1) return fail(PQB_ERR_ARG, "batch count must be positive");
+    if (rows < 1 || cols < 1 || rows >= cols + 2)
+        return fail(PQB_ERR_ARG, "the device entry points need 1 <= rows <= cols + 1");
+    if (cols > PQB_MAX_COLS)
+        return fail(PQB_ERR_SIZE, "matrix has more than " + std::to_string(PQB_MAX_COLS) + " columns");
+    return PQB_OK;
+}
+
+// Launch plan of a batch of `count` equal-size matrices: the single-matrix
+// plan for the resident waves divided among the matrices.
+int plain_device_blocks(int64_t count, int R, int C, PlainPlan* plan, int64_t* blocks_per_matrix) {
+    int64_t wr;
+    {
+        std::lock_guard<std::mutex> lk(g_eng.mu);
+        int st = g_eng.ensure_init_locked(-1);
+        if (st) return st;
+        wr = g_eng.resident_waves_locked(C);
+    }
+    const PlainPlan pl = make_plain_plan(R, C, std::max<int64_t>(1, wr / count));
+    if (pl.blocks > kMaxBlocks) return fail(PQB_ERR_SIZE, "launch plan exceeds the workspace layout");
+    if (plan) *plan = pl;
+    *blocks_per_matrix = pl.blocks;
+    return PQB_OK;
+}
+
+int plain_device_run(const double* dA, int64_t count, int64_t rows, int64_t cols, double* d_out, void* d_work,
+                     size_t work_bytes, hipStream_t st, bool single) {
+    if (!dA || !d_out || !d_work) return fail(PQB_ERR_ARG, "NULL device pointer");
+    int s = plain_device_check(count, rows, cols);
+    if (s) return s;
+    const int R = (int)rows, C = (int)cols;
+    PlainPlan pl;
+    int64_t bpm = 0;
+    s = plain_device_blocks(count, R, C, &pl, &bpm);
+    if (s) return s;
+    if (single) bpm = kMaxBlocks;  // the documented single-matrix layout
+    if (work_bytes < plain_device_bytes(count, rows, cols, bpm)) return fail(PQB_ERR_ARG, "workspace too small");
+    char* w = static_cast<char*>(d_work);
+    unsigned* counters = reinterpret_cast<unsigned*>(w);
+    ddc* partials = reinterpret_cast<ddc*>(w + round256((size_t)count * sizeof(unsigned)));
+    double2* m2 = reinterpret_cast<double2*>(reinterpret_cast<char*>(partials) + (size_t)count * bpm * sizeof(ddc));
+    const int64_t m2n = (int64_t)2 * R * C;
+    hipError_t e = launch_plain_prep(reinterpret_cast<const double2*>(dA), m2, R * C, count, st);
+    if (e != hipSuccess) return hip_fail(e, "prep launch");
+    constexpr int64_t kMaxGridY = 65535;
+    for (int64_t off = 0; off < count; off += kMaxGridY) {
+        const int64_t nt = std::min(kMaxGridY, count - off);
+        PlainParams p;
+        p.M2 = m2 + off * m2n;
+        p.m2_stride = m2n;
+        p.R = R;
+        p.K = pl.K;
+        p.B = pl.B;
+        p.H = pl.H;
+        p.partials = partials + off * pl.blocks;
+        p.counter = counters + off;
+        p.out = d_out + 6 * off;
+        p.scale = std::ldexp(1.0, -(R - 1));
+        e = launch_plain(C, p, pl.blocks, pl.block, (int)nt, st);
+        if (e != hipSuccess) return hip_fail(e, "plain sweep launch");
+    }
+    return PQB_OK;
+}
+
 // Repeated-row permanent: a batch of one task (batch.hip applies the
 // reference's validation and edge rules).
 int repeated_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* row_mult,
@@ -453,45 +533,29 @@ int pqb_exact_bound(const double* A, int64_t rows, int64_t cols, int64_t ld, dou
 
 size_t pqb_perm_device_workspace(int64_t rows, int64_t cols) {
     if (rows < 1 || cols < 1) return kCounterBytes;
-    return kCounterBytes + (size_t)kMaxBlocks * sizeof(ddc) + (size_t)2 * rows * cols * sizeof(double2);
+    return plain_device_bytes(1, rows, cols, kMaxBlocks);
 }
 
 int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out, void* d_work,
                     size_t work_bytes, void* stream) {
-    if (!dA || !d_out || !d_work) return fail(PQB_ERR_ARG, "NULL device pointer");
-    if (rows < 1 || cols < 1 || rows >= cols + 2)
-        return fail(PQB_ERR_ARG, "pqb_perm_device needs 1 <= rows <= cols + 1");
-    if (cols > PQB_MAX_COLS)
-        return fail(PQB_ERR_SIZE, "matrix has more than " + std::to_string(PQB_MAX_COLS) + " columns");
-    if (work_bytes < pqb_perm_device_workspace(rows, cols)) return fail(PQB_ERR_ARG, "workspace too small");
-    const int R = (int)rows, C = (int)cols;
-    int64_t wr;
-    {
-        std::lock_guard<std::mutex> lk(g_eng.mu);
-        int st = g_eng.ensure_init_locked(-1);
-        if (st) return st;
-        wr = g_eng.resident_waves_locked(C);
-    }
-    const PlainPlan pl = make_plain_plan(R, C, wr);
-    if (pl.blocks > kMaxBlocks) return fail(PQB_ERR_SIZE, "launch plan exceeds the workspace layout");
-    char* w = static_cast<char*>(d_work);
-    double2* m2 = reinterpret_cast<double2*>(w + kCounterBytes + (size_t)kMaxBlocks * sizeof(ddc));
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    hipError_t e = launch_plain_prep(reinterpret_cast<const double2*>(dA), m2, R * C, st);
-    if (e != hipSuccess) return hip_fail(e, "prep launch");
-    PlainParams p;
-    p.M2 = m2;
-    p.R = R;
-    p.K = pl.K;
-    p.B = pl.B;
-    p.H = pl.H;
-    p.partials = reinterpret_cast<ddc*>(w + kCounterBytes);
-    p.counter = reinterpret_cast<unsigned*>(w);
-    p.out = d_out;
-    p.scale = std::ldexp(1.0, -(R - 1));
-    e = launch_plain(C, p, pl.blocks, pl.block, st);
-    if (e != hipSuccess) return hip_fail(e, "plain sweep launch");
+    return plain_device_run(dA, 1, rows, cols, d_out, d_work, work_bytes, static_cast<hipStream_t>(stream), true);
+}
+
+int pqb_perm_batch_device_workspace(int64_t count, int64_t rows, int64_t cols, size_t* bytes) {
+    if (!bytes) return fail(PQB_ERR_ARG, "bytes pointer is NULL");
+    int st = plain_device_check(count, rows, cols);
+    if (st) return st;
+    int64_t bpt = 0;
+    st = plain_device_blocks(count, (int)rows, (int)cols, nullptr, &bpt);
+    if (st) return st;
+    *bytes = plain_device_bytes(count, rows, cols, bpt);
     return PQB_OK;
+}
+
+int pqb_perm_batch_device(const double* dA, int64_t count, int64_t rows, int64_t cols, double* d_out,
+                          void* d_work, size_t work_bytes, void* stream) {
+    return plain_device_run(dA, count, rows, cols, d_out, d_work, work_bytes, static_cast<hipStream_t>(stream),
+                            false);
 }
 
 int pqb_plain_plan(int64_t rows, int64_t cols, int64_t waves_resident, int64_t* plan) {

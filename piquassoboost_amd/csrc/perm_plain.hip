@@ -117,6 +117,11 @@ template <int NC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 32 ? 3 : 1)))
 void plain_sweep_kernel(PlainParams p) {
     __shared__ ddc red[256 / kWave];
+    // batch of equal-size matrices: blockIdx.y is the matrix, gridDim.x blocks each
+    const unsigned task = blockIdx.y;
+    ddc* const partials = p.partials + (size_t)task * gridDim.x;
+    unsigned* const counter = p.counter + task;
+    double* const out = p.out + 6 * (size_t)task;
 
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -128,7 +133,7 @@ void plain_sweep_kernel(PlainParams p) {
     const int H = p.H;  // group bits: rows 1 .. R-1-K-B
     const unsigned rowb = NC * sizeof(double2);
     // byte offsets of the +2A / -2A rows: row r of +2A at r*rowb, of -2A at (R+r)*rowb
-    const char* __restrict__ Mb = reinterpret_cast<const char*>(p.M2);
+    const char* __restrict__ Mb = reinterpret_cast<const char*>(p.M2 + (size_t)task * p.m2_stride);
     const unsigned neg = (unsigned)R * rowb;
     const unsigned low_p = (unsigned)(R - 1) * rowb;  // +2A[R-1]
     const unsigned low_m = neg + low_p;               // -2A[R-1]
@@ -218,9 +223,9 @@ void plain_sweep_kernel(PlainParams p) {
     // ---- grid reduction: last block sums the per-block partials ----
     __shared__ bool is_last;
     if (threadIdx.x == 0) {
-        p.partials[blockIdx.x] = v;
+        partials[blockIdx.x] = v;
         __threadfence();
-        const unsigned done = atomicAdd(p.counter, 1u);
+        const unsigned done = atomicAdd(counter, 1u);
         is_last = (done == gridDim.x - 1);
     }
     __syncthreads();
@@ -228,36 +233,39 @@ void plain_sweep_kernel(PlainParams p) {
         __threadfence();
         ddc s = {0.0, 0.0, 0.0, 0.0};
         for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-            const ddc q = p.partials[b];
+            const ddc q = partials[b];
             ddc_add(s, q);
         }
         s = block_reduce(s, red);
         if (threadIdx.x == 0) {
             s = ddc_scale(s, p.scale);
-            p.out[0] = s.hr + s.lr;
-            p.out[1] = s.hi + s.li;
-            p.out[2] = s.hr;
-            p.out[3] = s.lr;
-            p.out[4] = s.hi;
-            p.out[5] = s.li;
-            *p.counter = 0u;  // leave the counter ready for the next launch
+            out[0] = s.hr + s.lr;
+            out[1] = s.hi + s.li;
+            out[2] = s.hr;
+            out[3] = s.lr;
+            out[4] = s.hi;
+            out[5] = s.li;
+            *counter = 0u;  // leave the counter ready for the next launch
         }
     }
 }
 
-// M2 = [+2A ; -2A] from a packed R x NC device matrix.
-__global__ void plain_prep_kernel(const double2* __restrict__ A, double2* __restrict__ M2, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+// M2 = [+2A ; -2A] for each of `count` packed matrices of n elements.
+__global__ void plain_prep_kernel(const double2* __restrict__ A, double2* __restrict__ M2, int n, int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) {
+        const int64_t t = i / n, e = i - t * n;
         const double2 v = A[i];
-        M2[i] = make_double2(2.0 * v.x, 2.0 * v.y);
-        M2[n + i] = make_double2(-2.0 * v.x, -2.0 * v.y);
+        M2[2 * t * n + e] = make_double2(2.0 * v.x, 2.0 * v.y);
+        M2[2 * t * n + n + e] = make_double2(-2.0 * v.x, -2.0 * v.y);
     }
 }
 
-hipError_t launch_plain_prep(const double2* A, double2* M2, int n, hipStream_t st) {
+hipError_t launch_plain_prep(const double2* A, double2* M2, int n, int64_t count, hipStream_t st) {
     const int bd = 256;
-    hipLaunchKernelGGL(plain_prep_kernel, dim3((n + bd - 1) / bd), dim3(bd), 0, st, A, M2, n);
+    const int64_t total = (int64_t)n * count;
+    hipLaunchKernelGGL(plain_prep_kernel, dim3((unsigned)((total + bd - 1) / bd)), dim3(bd), 0, st, A, M2, n,
+                       total);
     return hipGetLastError();
 }
 
@@ -349,8 +357,8 @@ __global__ __launch_bounds__(256) void plain_exact_kernel(ExactParams p) {
 // ------------------------------------------------------------ dispatch --
 
 template <int NC>
-static hipError_t launch_plain_nc(const PlainParams& p, int nblocks, int bdim, hipStream_t st) {
-    hipLaunchKernelGGL(plain_sweep_kernel<NC>, dim3(nblocks), dim3(bdim), 0, st, p);
+static hipError_t launch_plain_nc(const PlainParams& p, int nblocks, int bdim, int ntasks, hipStream_t st) {
+    hipLaunchKernelGGL(plain_sweep_kernel<NC>, dim3(nblocks, ntasks), dim3(bdim), 0, st, p);
     return hipGetLastError();
 }
 
@@ -370,10 +378,10 @@ static const void* plain_fn_nc() {
     X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30)    \
     X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39) X(40)
 
-hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, hipStream_t st) {
+hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, int ntasks, hipStream_t st) {
     switch (nc) {
 #define PQB_CASE(N) \
-    case N: return launch_plain_nc<N>(p, nblocks, bdim, st);
+    case N: return launch_plain_nc<N>(p, nblocks, bdim, ntasks, st);
         PQB_NC_CASES(PQB_CASE)
 #undef PQB_CASE
         default: return hipErrorInvalidValue;

@@ -9,16 +9,18 @@ namespace pqb {
 
 struct ddc;
 
-// One plain permanent (R rows, NC columns, R <= NC + 1).
+// Plain permanents (R rows, NC columns, R <= NC + 1), one per grid row
+// (blockIdx.y = matrix index; a single matrix is a grid of height 1).
 struct PlainParams {
-    const double2* M2;   // 2R x NC, packed: rows [0,R) = +2A, rows [R,2R) = -2A
+    const double2* M2;   // per matrix 2R x NC, packed: rows [0,R) = +2A, rows [R,2R) = -2A
+    int64_t m2_stride;   // elements between consecutive matrices' M2 (2 R NC)
     int R;               // rows
     int K;               // Gray bits walked inside a chunk (rows R-K .. R-1)
     int B;               // lane bits (rows R-K-B .. R-K-1), B <= 6
     int H;               // group bits (rows 1 .. R-1-K-B), K + B + H = R - 1
-    ddc* partials;       // one dd complex per block
-    unsigned* counter;   // zero on entry; the last block restores it to zero
-    double* out;         // [re, im, hi_re, lo_re, hi_im, lo_im]
+    ddc* partials;       // one dd complex per block, gridDim.x per matrix
+    unsigned* counter;   // one per matrix, zero on entry; the last block restores it to zero
+    double* out;         // 6 per matrix: [re, im, hi_re, lo_re, hi_im, lo_im]
     double scale;        // 2^-(R-1)
 };
 
@@ -110,8 +112,8 @@ const void* grad_kernel_fn(int M, int K);
 hipError_t launch_grad(int M, int K, const GradParams& p, int nblocks, size_t lds_bytes, hipStream_t st);
 hipError_t launch_grad_reduce(const GradReduceParams& p, hipStream_t st);
 
-hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, hipStream_t st);
-hipError_t launch_plain_prep(const double2* A, double2* M2, int n, hipStream_t st);
+hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, int ntasks, hipStream_t st);
+hipError_t launch_plain_prep(const double2* A, double2* M2, int n, int64_t count, hipStream_t st);
 hipError_t launch_plain_exact(int nc, const ExactParams& p, int nblocks, int bdim, hipStream_t st);
 const void* plain_kernel_fn(int nc);
 
