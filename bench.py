@@ -106,8 +106,9 @@ def batched_leg() -> dict:
 
 
 def grad_leg(reps: int = 10) -> dict:
-    """Gradient workload: 256 n=20 and 8 n=26 blocks of a seeded 50-mode Haar
-    unitary (seed 1006), inputs resident in HBM.  Forward = the plain sweep of
+    """Gradient workload: 256 n=20 and 8 n=26 complex128 blocks, block b the
+    top-left corner of the 50-mode Haar unitary of seed 1006 + b, inputs
+    resident in HBM.  Forward = the plain sweep of
     the 264 permanents (pqb_perm_batch_device, one launch per size).
     Forward+backward = the adjoint sweep, which yields the permanents and every
     d perm / d a_ij (one launch per size plus its reduction).  Both timed with
@@ -118,7 +119,7 @@ def grad_leg(reps: int = 10) -> dict:
     from piquassoboost_amd.autograd import permanent
     from piquassoboost_amd.sampling.utilities import gradient_blocks
 
-    U, blocks, index = gradient_blocks()
+    blocks, _ = gradient_blocks()
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
 
@@ -185,7 +186,7 @@ def grad_leg(reps: int = 10) -> dict:
     steps = sum(B.shape[0] * 2.0 ** (n - 1) for n, B in blocks.items())
     flops = sum(B.shape[0] * 2.0 ** (n - 1) * (24 * n - 22) for n, B in blocks.items())
     return {
-        "workload": "256 n=20 + 8 n=26 complex128 blocks of a 50-mode Haar unitary (seed 1006), "
+        "workload": "256 n=20 + 8 n=26 complex128 top-left blocks of 50-mode Haar unitaries (seeds 1006+b), "
                     "loss sum log|perm|^2",
         "gray_steps": steps,
         "forward_plain_ms": fwd_ms,

@@ -1,0 +1,164 @@
+"""Generalized Clifford & Clifford boson sampling (Brod and Oszmaniec 2020) as
+the reference runs it, with the permanents of each photon step on the GPU.
+
+Mirrors piquassoboost.sampling.simulation_strategies.GeneralizedCliffordsSimulationStrategy
+(the Python classes :33-66) over the reference's C++ strategies:
+
+* ``GeneralizedCliffordsBSimulationStrategy`` -- CGeneralizedCliffordsBSimulationStrategy.cpp:
+  per sample, photons enter one at a time in a random order (update_input_by_single_photon,
+  samplingHelperFunctions.cpp:370-399); after each, the output distribution over modes for one
+  more photon is computed (compute_pmf, :78-342) and an output mode is drawn from it
+  (sample_from_pmf, :350-367); fill_r_sample :281-305, simulate :135-275.
+* ``GeneralizedCliffordsBUniformLossesSimulationStrategy`` --
+  CGeneralizedCliffordsBUniformLossesSimulationStrategy.cpp: the number of surviving photons is
+  drawn first from Binomial(n, transmissivity^2) (:285-316), then the same photon loop.
+
+compute_pmf needs, for every occupied input mode, the permanent of the effective matrix
+with one photon removed from that mode (the reference calls
+BBFGPermanentCalculatorRepeated once per mode); here all of them go to the engine as one
+batched launch.  The random numbers come from the shared std::mt19937 restated in
+``piquassoboost_amd.sampling.mt19937`` (seeded as the reference does), so seeded runs give
+the reference's samples.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from ... import _capi as C
+from ..mt19937 import MT19937
+
+# one generator shared by every strategy, as pic::rng_gen (samplingHelperFunctions.cpp:56-60)
+rng_gen = MT19937()
+
+
+def seed_random_generator(value: int) -> None:
+    rng_gen.seed(int(value))
+
+
+def modes_state_to_particle_state(mode_state) -> np.ndarray:
+    """Mode occupations -> list of the modes of the particles
+    (CGeneralizedCliffordsBSimulationStrategy.cpp:322-336)."""
+    s = np.asarray(mode_state, dtype=np.int64)
+    return np.repeat(np.arange(s.shape[0], dtype=np.int64), s)
+
+
+def compute_pmf(interferometer_matrix, sample, current_input, permanent_batch=None) -> np.ndarray:
+    """Probability of each output mode receiving the next photon
+    (samplingHelperFunctions.cpp:78-342).
+
+    For each occupied input mode idx: permanent_addends[idx] = perm of the effective matrix
+    (rows: occupied output modes of `sample`, columns: occupied input modes after removing
+    one photon from idx, with their multiplicities).  Then
+    pmf[mdx] = |sum_idx current_input[idx] * addend[idx] * U[mdx, idx]|^2, normalised.
+    `permanent_batch` (default: the GPU engine) takes (matrix, row_mult, col_mult) tasks."""
+    U = np.asarray(interferometer_matrix)
+    sample = np.asarray(sample, dtype=np.int64)
+    current_input = np.asarray(current_input, dtype=np.int64)
+    perm_fn = C.permanent_batch if permanent_batch is None else permanent_batch
+    col_idx = np.flatnonzero(current_input > 0)
+    out_rows = np.flatnonzero(sample > 0)
+    tasks = []
+    for idx in col_idx:
+        inp = current_input.copy()
+        inp[idx] -= 1
+        in_cols = np.flatnonzero(inp > 0)
+        A = np.ascontiguousarray(U[np.ix_(out_rows, in_cols)], dtype=np.complex128)
+        tasks.append((A, sample[out_rows], inp[in_cols]))
+    addends = np.asarray(perm_fn(tasks), dtype=np.complex128) if tasks else np.zeros(0, np.complex128)
+    perms = np.zeros(sample.shape[0], dtype=np.complex128)
+    for k, idx in enumerate(col_idx):  # the reference's order of accumulation
+        perms = perms + (current_input[idx] * addends[k]) * U[:, idx]
+    pmf = perms.real * perms.real + perms.imag * perms.imag
+    return pmf / pmf.sum()
+
+
+def sample_from_pmf(sample: np.ndarray, pmf: np.ndarray) -> None:
+    """samplingHelperFunctions.cpp:350-367: first index whose running sum reaches the draw
+    (index 0 when rounding leaves the total below it)."""
+    rand_num = rng_gen.random_double_0_1()
+    sampled_index = 0
+    prob_sum = 0.0
+    for idx in range(pmf.shape[0]):
+        prob_sum = prob_sum + float(pmf[idx])
+        if prob_sum >= rand_num:
+            sampled_index = idx
+            break
+    sample[sampled_index] += 1
+
+
+def update_input_by_single_photon(current_input: np.ndarray, working_input_state: np.ndarray) -> np.ndarray:
+    """samplingHelperFunctions.cpp:370-399: move a random remaining particle into the
+    input; returns the reduced particle list."""
+    if working_input_state.shape[0] == 0:
+        raise Exception(
+            "Generalized Cliffords B Simulation Strategy::update_input_by_single_photon: "
+            "the size of working_input_state is zero"
+        )
+    rand_index = rng_gen.random_index(working_input_state.shape[0])
+    current_input[working_input_state[rand_index]] += 1
+    return np.delete(working_input_state, rand_index)
+
+
+class GeneralizedCliffordsBSimulationStrategy:
+    """GeneralizedCliffordsSimulationStrategy.py:33-47 /
+    CGeneralizedCliffordsBSimulationStrategy.cpp (lossless)."""
+
+    def __init__(self, interferometer_matrix, seed=None, lib: int = 0, permanent_batch=None):
+        self.interferometer_matrix = np.ascontiguousarray(interferometer_matrix, dtype=np.complex128)
+        self.lib = lib
+        self._perm = permanent_batch
+        seed_random_generator(int(time.time()))  # the C++ constructor seeds with time(NULL)
+        if seed is not None:
+            self.seed(seed)
+
+    def seed(self, value) -> None:
+        seed_random_generator(value)
+
+    def _fill_r_sample(self, sample, current_input, working, photons) -> None:
+        while photons > int(sample.sum()):
+            working = update_input_by_single_photon(current_input, working)
+            pmf = compute_pmf(self.interferometer_matrix, sample, current_input, self._perm)
+            sample_from_pmf(sample, pmf)
+
+    def _photons(self, input_state) -> int:
+        return int(np.sum(input_state))
+
+    def simulate(self, input_state, samples_number: int = 1):
+        """Tuple of `samples_number` output states (mode occupations, int64)."""
+        input_state = np.asarray(input_state, dtype=np.int64)
+        particles = modes_state_to_particle_state(input_state)
+        samples = []
+        for _ in range(max(0, int(samples_number))):
+            sample = np.zeros(input_state.shape[0], dtype=np.int64)
+            current_input = np.zeros(input_state.shape[0], dtype=np.int64)
+            self._fill_r_sample(sample, current_input, particles.copy(), self._photons(input_state))
+            samples.append(sample)
+        return tuple(samples)
+
+
+class GeneralizedCliffordsBUniformLossesSimulationStrategy(GeneralizedCliffordsBSimulationStrategy):
+    """GeneralizedCliffordsSimulationStrategy.py:50-64 /
+    CGeneralizedCliffordsBUniformLossesSimulationStrategy.cpp: transmissivity t, each photon
+    survives with probability t^2."""
+
+    def __init__(self, interferometer_matrix, transmissivity, seed=None, lib: int = 0, permanent_batch=None):
+        self.transmissivity = float(transmissivity)
+        super().__init__(interferometer_matrix, seed=seed, lib=lib, permanent_batch=permanent_batch)
+
+    def _photons(self, input_state) -> int:
+        """calculate_current_photon_number (:285-296) over the binomial weights (:298-316)."""
+        n = int(np.sum(input_state))
+        eta = self.transmissivity * self.transmissivity
+        from math import comb
+
+        probs = [comb(n, i) * eta ** i * (1.0 - eta) ** (n - i) for i in range(n + 1)]
+        rand_num = rng_gen.random_double_0_1()
+        photon_number = 0
+        weight = probs[0]
+        while weight < rand_num and weight < 1.0:
+            photon_number += 1
+            weight += probs[photon_number]
+        return photon_number

@@ -51,26 +51,22 @@ def uneven_batch(seed: int = 1005, count: int = 4096, modes: int = 100, n_min: i
 
 
 def gradient_blocks(seed: int = 1006, modes: int = 50, sizes=((20, 256), (26, 8))):
-    """Square blocks of one Haar interferometer for gradient workloads.
-
-    For each (n, count) in `sizes`: `count` n x n submatrices U[rows, cols]
-    with rows and columns drawn as sorted random n-subsets of the `modes`
-    modes (seeded).  Returns (U, {n: array count x n x n, ...},
-    {n: [(rows, cols), ...]})."""
-    U = haar_unitary(modes, seed)
-    rng = np.random.default_rng(seed + 1)
-    blocks, index = {}, {}
+    """Square blocks for the gradient workload: block b (counted over `sizes`
+    in order) is the top-left n x n block of its own Haar unitary
+    haar_unitary(modes, seed + b).  Returns ({n: array count x n x n},
+    {n: [seed of each block's unitary]})."""
+    blocks, seeds = {}, {}
+    b = 0
     for n, count in sizes:
         B = np.empty((count, n, n), dtype=np.complex128)
-        idx = []
-        for b in range(count):
-            r = np.sort(rng.choice(modes, size=n, replace=False))
-            c = np.sort(rng.choice(modes, size=n, replace=False))
-            B[b] = U[np.ix_(r, c)]
-            idx.append((r, c))
+        s = []
+        for k in range(count):
+            B[k] = haar_unitary(modes, seed + b)[:n, :n]
+            s.append(seed + b)
+            b += 1
         blocks[n] = B
-        index[n] = idx
-    return U, blocks, index
+        seeds[n] = s
+    return blocks, seeds
 
 
 def adapt_interferometer(U, input_state, output_state) -> np.ndarray:

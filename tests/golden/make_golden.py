@@ -147,7 +147,37 @@ def repeated_c3():
                         row_mult=rows, col_mult=cols, value_ld=np.array([v.real, v.imag]), value_dd=np.array(dp))
 
 
-STEPS = {"kat": kat, "haar": haar_blocks, "bernoulli": bernoulli, "c3": repeated_c3}
+def seeding():
+    """Seeded boson-sampling runs of unittests/test_seeding.py (data of the reference's
+    tests): the 5-mode interferometer (:30-49), input state [2,1,1,0,1], seed 123, 20 shots;
+    expected samples of the lossless run (:63-84) and of the uniformly lossy one with
+    singular values 0.9 (:134-155).  The non-uniform lossy run (:97-118) is not stored: its
+    2m-mode dilation is built by piquasso, which is not part of the reference."""
+    U = [[-0.17959207 - 0.29175972j, -0.64550941 - 0.02897055j, -0.023922 - 0.38854671j,
+          -0.07908932 + 0.35617293j, -0.39779191 + 0.14902272j],
+         [-0.36896208 + 0.19468375j, -0.11545557 + 0.20434514j, 0.25548079 + 0.05220164j,
+          -0.51002161 + 0.38442256j, 0.48106678 - 0.25210091j],
+         [0.25912844 + 0.16131742j, 0.11886251 + 0.12632645j, 0.69028213 - 0.25734432j,
+          0.01276639 + 0.05841739j, 0.03713264 + 0.57364845j],
+         [-0.20314019 - 0.18973473j, 0.59146854 + 0.28605532j, -0.11096495 - 0.26870144j,
+          -0.47290354 - 0.0489408j, -0.42459838 + 0.01554643j],
+         [-0.5021973 - 0.53474291j, 0.24524545 - 0.0741398j, 0.37786104 + 0.10225255j,
+          0.46696955 + 0.10636677j, 0.07171789 - 0.09194236j]]
+    lossless = [(4, 0, 0, 0, 1), (0, 0, 0, 5, 0), (0, 1, 0, 3, 1), (0, 1, 1, 2, 1), (1, 0, 2, 0, 2),
+                (1, 0, 2, 2, 0), (4, 0, 0, 1, 0), (0, 0, 2, 2, 1), (0, 4, 0, 0, 1), (0, 0, 3, 0, 2),
+                (2, 0, 3, 0, 0), (1, 2, 1, 0, 1), (0, 0, 2, 1, 2), (0, 2, 1, 1, 1), (0, 1, 1, 1, 2),
+                (2, 1, 1, 0, 1), (0, 3, 1, 1, 0), (0, 0, 3, 1, 1), (4, 0, 0, 0, 1), (0, 0, 3, 1, 1)]
+    uniform = [(0, 1, 1, 2, 0), (0, 0, 1, 1, 1), (1, 0, 4, 0, 0), (1, 0, 2, 1, 0), (0, 1, 2, 0, 0),
+               (0, 0, 5, 0, 0), (1, 0, 1, 0, 2), (0, 1, 1, 1, 0), (0, 0, 2, 3, 0), (0, 0, 1, 2, 0),
+               (0, 0, 0, 1, 2), (0, 0, 1, 2, 1), (2, 1, 1, 0, 1), (2, 0, 2, 0, 0), (0, 2, 1, 0, 0),
+               (1, 0, 0, 2, 1), (3, 0, 0, 0, 1), (2, 1, 1, 0, 1), (0, 0, 1, 2, 2), (2, 1, 1, 0, 1)]
+    data = dict(U=[[_c(v) for v in row] for row in U], input_state=[2, 1, 1, 0, 1], seed=123, shots=20,
+                uniform_singular_value=0.9, lossless=lossless, uniform_losses=uniform)
+    (HERE / "seeding.json").write_text(json.dumps(data, indent=1))
+    print("seeding.json")
+
+
+STEPS = {"kat": kat, "haar": haar_blocks, "bernoulli": bernoulli, "c3": repeated_c3, "seeding": seeding}
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()

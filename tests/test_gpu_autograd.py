@@ -5,10 +5,10 @@
   CPU, complex128), for holomorphic and non-holomorphic losses and batch
   dimensions.
 * Chaining: phase gradients of a Clements mesh (m = 8) through both
-  permanents agree; for the gradient workload (256 n=20 and 8 n=26 blocks of
-  a Clements-decomposed 50-mode Haar unitary, loss sum log|perm|^2), the
-  phase gradient agrees with central finite differences (h = 1e-6) of the
-  loss evaluated with the forward pass."""
+  permanents agree; for blocks of the gradient workload (top-left n=20 and
+  n=26 blocks of Clements-decomposed 50-mode Haar unitaries, loss
+  sum log|perm|^2), the phase gradient agrees with central finite
+  differences (h = 1e-6) of the loss evaluated with the forward pass."""
 
 import numpy as np
 import pytest
@@ -93,41 +93,49 @@ def test_clements_chain_matches_torch_ryser():
 
 
 def test_c5_clements_chain_finite_differences():
-    """Gradient workload: 256 n=20 and 8 n=26 blocks of the 50-mode Haar
-    unitary, rebuilt from its Clements phases on the GPU; autograd phase
-    gradient against central differences of the loss (forward pass only)."""
-    U0, blocks, index = gradient_blocks()
-    M = decompose(U0)
-    th = torch.tensor(M.theta, device=DEV, requires_grad=True)
-    ph = torch.tensor(M.phi, device=DEV, requires_grad=True)
-    al = torch.tensor(M.alpha, device=DEV, requires_grad=True)
+    """Gradient workload blocks b = 0 (n = 20) and b = 256 (n = 26): the
+    50-mode Haar unitaries of seeds 1006 and 1262, Clements-decomposed and
+    rebuilt on the GPU; loss = sum log|perm(top-left block)|^2; autograd
+    phase gradients against central differences of the loss (h = 1e-6,
+    forward pass only)."""
+    blocks, seeds = gradient_blocks()
+    cases = [(20, seeds[20][0], blocks[20][0]), (26, seeds[26][0], blocks[26][0])]
+    meshes = [decompose(haar_unitary(50, s)) for _, s, _ in cases]
+    params = [[torch.tensor(getattr(M, k), device=DEV, requires_grad=True) for k in ("theta", "phi", "alpha")]
+              for M in meshes]
 
-    def loss_of(t, p, a):
-        U = unitary_torch(M.modes, t, p, a)
-        return sum(torch.log(permanent(_blocks(U, index[n])).abs() ** 2).sum() for n in sorted(index))
+    def loss_of(ps):
+        total = 0.0
+        for (n, _, _), M, (t, p, a) in zip(cases, meshes, ps):
+            U = unitary_torch(M.modes, t, p, a)
+            total = total + torch.log(permanent(U[:n, :n]).abs() ** 2)
+        return total
 
-    loss = loss_of(th, ph, al)
-    # the rebuilt unitary reproduces the Haar blocks
-    with torch.no_grad():
-        U = unitary_torch(M.modes, th, ph, al)
-        assert (_blocks(U, index[20]).cpu() - torch.tensor(blocks[20])).abs().max() < 1e-12
+    with torch.no_grad():  # the rebuilt unitaries reproduce the workload's blocks
+        for (n, _, B), M, (t, p, a) in zip(cases, meshes, params):
+            assert (unitary_torch(M.modes, t, p, a)[:n, :n].cpu() - torch.tensor(B)).abs().max() < 1e-12
+    loss = loss_of(params)
     print(f"C5 chain: loss={loss.item():.6f}", flush=True)
     loss.backward()
-    g = {"theta": th.grad.cpu().numpy(), "phi": ph.grad.cpu().numpy(), "alpha": al.grad.cpu().numpy()}
     rng = np.random.default_rng(55)
     h = 1e-6
-    base = {"theta": M.theta, "phi": M.phi, "alpha": M.alpha}
     worst = 0.0
-    for name, count in (("theta", 3), ("phi", 3), ("alpha", 2)):
-        for k in rng.choice(len(base[name]), size=count, replace=False):
-            vals = []
-            for sgn in (1, -1):
-                arrs = {kk: v.copy() for kk, v in base.items()}
-                arrs[name][k] += sgn * h
-                with torch.no_grad():
-                    vals.append(loss_of(*(torch.tensor(arrs[kk], device=DEV) for kk in ("theta", "phi", "alpha"))).item())
-            fd = (vals[0] - vals[1]) / (2 * h)
-            err = abs(fd - g[name][k]) / max(abs(g[name][k]), 1.0)
-            worst = max(worst, err)
-            assert err < 1e-5, (name, int(k), fd, g[name][k])
+    for m, M in enumerate(meshes):
+        base = [getattr(M, k) for k in ("theta", "phi", "alpha")]
+        for which, count in ((0, 3), (1, 3), (2, 2)):
+            for k in rng.choice(len(base[which]), size=count, replace=False):
+                vals = []
+                for sgn in (1, -1):
+                    ps = [[torch.tensor(getattr(Mm, kk), device=DEV) for kk in ("theta", "phi", "alpha")]
+                          for Mm in meshes]
+                    arr = base[which].copy()
+                    arr[k] += sgn * h
+                    ps[m][which] = torch.tensor(arr, device=DEV)
+                    with torch.no_grad():
+                        vals.append(loss_of(ps).item())
+                fd = (vals[0] - vals[1]) / (2 * h)
+                g = params[m][which].grad[k].item()
+                err = abs(fd - g) / max(abs(g), 1.0)
+                worst = max(worst, err)
+                assert err < 1e-5, (m, which, int(k), fd, g)
     print(f"C5 chain finite differences: worst {worst:.3e}", flush=True)

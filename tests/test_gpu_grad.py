@@ -37,8 +37,9 @@ def perm_oracle(A, prec="double"):
 
 @pytest.fixture(scope="module")
 def c5():
-    U, blocks, index = gradient_blocks()
-    return U, blocks, index
+    """256 n=20 and 8 n=26 top-left blocks of the Haar unitaries of seeds 1006 + b."""
+    blocks, seeds = gradient_blocks()
+    return blocks
 
 
 @pytest.mark.parametrize("n", list(range(1, 13)))
@@ -66,7 +67,7 @@ def test_real_integer_matrix_exact():
 
 
 def test_c5_n20_all_minors(c5):
-    U, blocks, index = c5
+    blocks = c5
     A_all = blocks[20][:2]
     perms, G = C.permanent_grad_batch(A_all)
     worst = 0.0
@@ -83,7 +84,7 @@ def test_c5_n20_all_minors(c5):
 
 
 def test_c5_n26_sampled_minors(c5):
-    U, blocks, index = c5
+    blocks = c5
     A = blocks[26][0]
     p, G = C.permanent_grad(A)
     rng = np.random.default_rng(5)
@@ -98,14 +99,15 @@ def test_c5_n26_sampled_minors(c5):
 
 
 def test_c5_finite_differences(c5):
-    """64 random entries of an n=20 block, h = 1e-6: perm is linear in each
-    entry, so the central difference of perm equals G_ij up to the rounding of
-    the two permanents (double-double oracle); the loss L = log|perm|^2 has
-    dL/dRe a = 2 Re(conj(p) G)/|p|^2 and dL/dIm a = -2 Im(conj(p) G)/|p|^2."""
-    U, blocks, index = c5
-    A = blocks[20][3]
+    """64 entries of the first n=20 block picked with seed 1007, h = 1e-6: perm
+    is linear in each entry, so the central difference of perm equals G_ij up
+    to the rounding of the two permanents (double-double oracle); the loss
+    L = log|perm|^2 has dL/dRe a = 2 Re(conj(p) G)/|p|^2 and
+    dL/dIm a = -2 Im(conj(p) G)/|p|^2."""
+    blocks = c5
+    A = blocks[20][0]
     p, G = C.permanent_grad(A)
-    rng = np.random.default_rng(64)
+    rng = np.random.default_rng(1007)
     h = 1e-6
     ents = [tuple(int(x) for x in rng.integers(0, 20, size=2)) for _ in range(64)]
     worst_p, worst_l = 0.0, 0.0
@@ -142,7 +144,7 @@ def laplace_check(A, p, G, rows, rng, tol):
 def test_c5_full_batch_laplace(c5):
     """The whole gradient workload (256 n=20 and 8 n=26 blocks): perms against
     the plain sweep and the Laplace expansion along row 0 and column 0."""
-    U, blocks, index = c5
+    blocks = c5
     for n, B in sorted(blocks.items()):
         print(f"C5 batch n={n} x {B.shape[0]}", flush=True)
         perms, G = C.permanent_grad_batch(B)

@@ -1,0 +1,108 @@
+"""Boson-sampling strategies, host side (no device): the std::mt19937
+restatement against the C++ standard's known answer and numpy's legacy
+Mersenne Twister; compute_pmf against the Laplace identity it relies on; and
+the photon-by-photon sampler, run with the oracle's permanents, against the
+seeded samples of the reference's unittests/test_seeding.py (fixture
+tests/golden/seeding.json).  The reference's samples reach the test through
+piquasso, which reorders them; the comparison is therefore of the 20-sample
+multisets, which a wrong pmf or random-number use would not reproduce."""
+
+import json
+from collections import Counter
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from piquassoboost_amd.sampling.mt19937 import MT19937
+from piquassoboost_amd.sampling.simulation_strategies.GeneralizedCliffordsSimulationStrategy import (
+    GeneralizedCliffordsBSimulationStrategy,
+    GeneralizedCliffordsBUniformLossesSimulationStrategy,
+    compute_pmf,
+    modes_state_to_particle_state,
+)
+
+
+def oracle_batch(tasks):
+    return [1.0 + 0j if A.size == 0 else oracle.bbfg_repeated(A, c, r, "dd") for A, r, c in tasks]
+
+
+@pytest.fixture(scope="module")
+def seeding():
+    d = json.loads((GOLDEN / "seeding.json").read_text())
+    U = np.array(d["U"], dtype=float)
+    d["U"] = U[..., 0] + 1j * U[..., 1]
+    return d
+
+
+def test_mt19937_standard_known_answer():
+    g = MT19937()  # default seed 5489
+    for _ in range(9999):
+        g()
+    assert g() == 4123659995  # [rand.predef]: 10000th invocation of a default-constructed mt19937
+
+
+def test_mt19937_matches_numpy_legacy_seeding():
+    bg = np.random.MT19937()
+    bg._legacy_seeding(123)
+    g = MT19937(123)
+    assert [g() for _ in range(1500)] == [int(x) for x in bg.random_raw(1500)]
+    g.seed(2**32 + 7)  # seed values are taken modulo 2^32
+    h = MT19937(7)
+    assert [g() for _ in range(10)] == [h() for _ in range(10)]
+
+
+def test_particle_state():
+    assert modes_state_to_particle_state([2, 1, 1, 0, 1]).tolist() == [0, 0, 1, 2, 4]
+
+
+def test_compute_pmf_is_the_next_photon_distribution():
+    """pmf[m] is proportional to |perm|^2 of the expanded matrix with output row m added
+    (the Laplace expansion along the new row that compute_pmf uses)."""
+    rng = np.random.default_rng(8)
+    m = 7
+    Z = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+    U, _ = np.linalg.qr(Z)
+    current_input = np.array([1, 0, 2, 0, 1, 1, 0])
+    sample = np.array([0, 2, 0, 1, 0, 1, 0])  # one photon fewer than the input
+    pmf = compute_pmf(U, sample, current_input, permanent_batch=oracle_batch)
+    cols = np.repeat(np.arange(m), current_input)
+    ref = np.empty(m)
+    for mdx in range(m):
+        s = sample.copy()
+        s[mdx] += 1
+        rows = np.repeat(np.arange(m), s)
+        ref[mdx] = abs(oracle.ryser(U[np.ix_(rows, cols)], "dd")) ** 2
+    ref /= ref.sum()
+    assert np.abs(pmf - ref).max() < 1e-13
+    assert abs(pmf.sum() - 1.0) < 1e-14
+
+
+def test_lossless_seeded_samples(seeding):
+    d = seeding
+    s = GeneralizedCliffordsBSimulationStrategy(d["U"], d["seed"], permanent_batch=oracle_batch)
+    got = [tuple(int(x) for x in v) for v in s.simulate(d["input_state"], d["shots"])]
+    assert Counter(got) == Counter(tuple(x) for x in d["lossless"])
+
+
+def test_uniform_losses_seeded_samples(seeding):
+    """LossyInterferometer U diag(0.9) U U^T: uniform singular values, so the strategy gets
+    the matrix divided by its largest singular value and that value as transmissivity
+    (the boosted measurement's uniform-loss branch)."""
+    d = seeding
+    U = d["U"]
+    lossy = U @ np.diag([d["uniform_singular_value"]] * 5) @ U @ U.T
+    sv = np.linalg.svd(lossy)[1]
+    s = GeneralizedCliffordsBUniformLossesSimulationStrategy(lossy / sv[0], sv[0], d["seed"],
+                                                             permanent_batch=oracle_batch)
+    got = [tuple(int(x) for x in v) for v in s.simulate(d["input_state"], d["shots"])]
+    assert Counter(got) == Counter(tuple(x) for x in d["uniform_losses"])
+
+
+def test_other_seed_gives_other_samples(seeding):
+    """The fixture comparison is sensitive: seed 124 does not reproduce the multiset."""
+    d = seeding
+    s = GeneralizedCliffordsBSimulationStrategy(d["U"], 124, permanent_batch=oracle_batch)
+    got = [tuple(int(x) for x in v) for v in s.simulate(d["input_state"], d["shots"])]
+    assert Counter(got) != Counter(tuple(x) for x in d["lossless"])
