@@ -199,6 +199,49 @@ def grad_leg(reps: int = 10) -> dict:
     }
 
 
+def sampling_leg(photons: int = 20, modes: int = 60, shots: int = 3) -> dict:
+    """Seeded generalized-Clifford boson sampling (20 photons in modes 0..19 of a 60-mode
+    Haar interferometer, seed 1010): time per sample with compute_pmf's permanents from one
+    column-minors sweep per photon step, from the per-mode batched permanents, and from
+    the oracle's restated reference sweep on the host (one call per mode, as the reference
+    does).  All three draw the same samples."""
+    from piquassoboost_amd.sampling.simulation_strategies.GeneralizedCliffordsSimulationStrategy import (
+        GeneralizedCliffordsBSimulationStrategy,
+    )
+    from piquassoboost_amd.sampling.utilities import haar_unitary
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+
+    U = haar_unitary(modes, 1010)
+    inp = np.zeros(modes, np.int64)
+    inp[:photons] = 1
+    threads = max(1, min(16, os.cpu_count() or 1))
+
+    def oracle_batch(tasks):
+        return [1.0 + 0j if A.size == 0 else oracle.bbfg_repeated(A, c, r, "double", threads=threads)
+                for A, r, c in tasks]
+
+    res = {}
+    samples = {}
+    for name, kw in (("minors", {"pmf_method": "minors"}), ("batch", {"pmf_method": "batch"}),
+                     ("host_reference", {"permanent_batch": oracle_batch, "pmf_method": "batch"})):
+        strat = GeneralizedCliffordsBSimulationStrategy(U, 1010, **kw)
+        t0 = time.perf_counter()
+        out = strat.simulate(inp, shots)
+        res[name] = (time.perf_counter() - t0) / shots
+        samples[name] = [tuple(int(x) for x in v) for v in out]
+    return {
+        "workload": f"{photons} photons in modes 0..{photons - 1} of a {modes}-mode Haar interferometer "
+                    f"(seed 1010), {shots} seeded samples",
+        "s_per_sample_minors": res["minors"],
+        "s_per_sample_batched": res["batch"],
+        "s_per_sample_host_reference": res["host_reference"],
+        "host_threads": threads,
+        "same_samples": samples["minors"] == samples["batch"] == samples["host_reference"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +250,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the forward+backward leg")
+    ap.add_argument("--no-sampling", action="store_true", help="skip the boson-sampling leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
                          "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
@@ -303,6 +347,8 @@ def main() -> int:
         line["batched"] = batched_leg()
     if not a.no_grad:
         line["grad"] = grad_leg()
+    if not a.no_sampling:
+        line["sampling"] = sampling_leg()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A)
     print(json.dumps(line), flush=True)

@@ -231,6 +231,15 @@ int pqb_perm_grad_workspace(int64_t count, int64_t n, size_t* bytes);
 int pqb_perm_grad_device(const double* dA, int64_t count, int64_t n, double* d_perm, double* d_grad,
                          void* d_work, size_t work_bytes, void* stream);
 
+/* Column minors of a rows x (rows+1) complex matrix: out[j] (2 doubles each,
+ * cols of them) = perm(A without column j), from one sweep over the rows' sign
+ * vectors with leave-one-out products of the column sums.  This is what
+ * compute_pmf needs per photon step (samplingHelperFunctions.cpp:137-172 calls
+ * BBFGPermanentCalculatorRepeated once per occupied input mode, each a minor of
+ * the expanded matrix); one call replaces those rows+1 sweeps.  rows = 0 gives
+ * out[0] = 1.  Host pointers, synchronous. */
+int pqb_perm_minors(const double* A, int64_t rows, int64_t cols, int64_t ld, double* out);
+
 /* Launch plan of the adjoint sweep: plan[0] = M (columns per lane), [1] = K
  * (register bits), [2] = L (lane bits), [3] = H (high bits), [4] = waves per
  * matrix, [5] = blocks, [6] = LDS bytes per block.  waves_resident > 0: plan

@@ -68,6 +68,7 @@ SIGNATURES = {
     "pqb_perm_grad_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "pqb_grad_plan": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p]),
+    "pqb_perm_minors": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp]),
     "pqb_perm_batch_device_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                        ctypes.POINTER(ctypes.c_size_t)]),
     "pqb_perm_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
@@ -86,6 +87,17 @@ class EngineError(Exception):
         self.code = code
 
 
+def _share_torch_hip_runtime() -> None:
+    """A process holds one HIP runtime: PyTorch ships its own libamdhip64, and if the engine
+    library is loaded first it binds the system one, after which PyTorch finds no GPU.
+    When PyTorch is installed it is therefore imported before the engine library, whose
+    libamdhip64 dependency then resolves to PyTorch's (same soname)."""
+    import importlib.util
+
+    if importlib.util.find_spec("torch") is not None:
+        import torch  # noqa: F401
+
+
 def lib():
     """Load libpqb_perm.so (raises if it has not been built)."""
     global _lib
@@ -97,6 +109,7 @@ def lib():
                         f"{LIB_PATH} is missing: build the HIP engine first "
                         "(python -m piquassoboost_amd.build)"
                     )
+                _share_torch_hip_runtime()
                 L = ctypes.CDLL(str(LIB_PATH))
                 for name, (res, args) in SIGNATURES.items():
                     fn = getattr(L, name)
@@ -339,3 +352,12 @@ def permanent_batch_device(dA_ptr: int, count: int, rows: int, cols: int, d_out_
                            work_bytes: int, stream: int = 0) -> None:
     """Asynchronous batch of plain permanents on device pointers (see pqb_perm_batch_device)."""
     _check(lib().pqb_perm_batch_device(dA_ptr, count, rows, cols, d_out_ptr, d_work_ptr, work_bytes, stream or None))
+
+
+def permanent_minors(A) -> np.ndarray:
+    """For a rows x (rows+1) matrix: [perm(A without column j) for j in range(rows + 1)]."""
+    M = as_matrix(A)
+    out = np.zeros(M.shape[1], dtype=np.complex128)
+    _check(lib().pqb_perm_minors(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
+                                 out.ctypes.data_as(_dp)))
+    return out

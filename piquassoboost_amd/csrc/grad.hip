@@ -1,7 +1,8 @@
 // Host side of the adjoint sweep (perm_grad.hip): launch plan, workspace
 // layout and the C-ABI entry points for the permanent together with its
 // gradient G_ij = d perm(A) / d a_ij of square complex matrices, one matrix or
-// a batch of equal-size matrices per call.
+// a batch of equal-size matrices per call; and for the column minors of an
+// n x (n+1) matrix (the same sweep without the row gradient).
 
 #include <hip/hip_runtime.h>
 
@@ -19,6 +20,8 @@
 namespace pqb {
 
 struct GradPlan {
+    bool minors = false;  // column-minors mode (n x (n+1) input)
+    int cols = 1;         // columns of the input
     int M = 1;            // columns per lane (4 lanes per sign vector)
     int K = 0;            // register bits (walked inside a block)
     int L = 0;            // delta-lane bits
@@ -33,25 +36,29 @@ struct GradPlan {
 // suffix products in 256 VGPRs (the instantiations of perm_grad.hip).
 static int grad_register_bits(int M) { return M <= 5 ? 3 : (M <= 7 ? 2 : 1); }
 
-static size_t grad_lds_bytes(int n, int M, int H) {
+// Staged matrix (n x 4M) plus, per wave, the 16 x 4M transpose area and (adjoint
+// mode) the H x 4M high-row accumulators.
+static size_t grad_lds_bytes(int n, int M, int H, bool minors) {
     const size_t NP = 4 * (size_t)M;
-    return sizeof(double2) * ((size_t)n * NP + 4 * (16 + (size_t)H) * NP);
+    return sizeof(double2) * ((size_t)n * NP + 4 * (16 + (minors ? 0 : (size_t)H)) * NP);
 }
 
-static GradPlan grad_shape(int n) {
+static GradPlan grad_shape(int n, bool minors) {
     GradPlan g;
-    g.M = (n + 3) / 4;
+    g.minors = minors;
+    g.cols = minors ? n + 1 : n;
+    g.M = (g.cols + 3) / 4;
     g.L = std::min(4, n - 1);
     g.K = std::min(grad_register_bits(g.M), n - 1 - g.L);
     g.H = n - 1 - g.L - g.K;
-    g.lds = grad_lds_bytes(n, g.M, g.H);
+    g.lds = grad_lds_bytes(n, g.M, g.H, minors);
     return g;
 }
 
 // Waves per task: about four rounds of the resident waves over the batch,
 // at least 4 (one block), at most the number of blocks of 2^K steps.
-static GradPlan make_grad_plan(int n, int64_t count, int64_t waves_resident) {
-    GradPlan g = grad_shape(n);
+static GradPlan make_grad_plan(int n, int64_t count, int64_t waves_resident, bool minors = false) {
+    GradPlan g = grad_shape(n, minors);
     const int64_t NB = (int64_t)1 << g.H;
     const int64_t target = std::max<int64_t>(4 * std::max<int64_t>(waves_resident, 1), 4);
     int64_t per = (target + count - 1) / count;
@@ -62,20 +69,35 @@ static GradPlan make_grad_plan(int n, int64_t count, int64_t waves_resident) {
     return g;
 }
 
-static size_t grad_partials_bytes(int n, int64_t count, int64_t wpt) {
-    return (size_t)count * (size_t)wpt * ((size_t)n * n + 1) * sizeof(double2);
+// One partial per block of 4 waves: n x n gradient + permanent, or the n+1 column minors.
+static size_t grad_partial_elems(int n, bool minors) { return minors ? (size_t)n + 1 : (size_t)n * n + 1; }
+
+static size_t grad_partials_bytes(int n, int64_t count, int64_t wpt, bool minors = false) {
+    return (size_t)count * (size_t)(wpt / 4) * grad_partial_elems(n, minors) * sizeof(double2);
 }
 
 // Resident waves of the adjoint kernel for this shape (engine lock held).
 static int grad_resident_waves_locked(const GradPlan& g, int64_t* waves) {
-    const void* fn = grad_kernel_fn(g.M, g.K);
+    const void* fn = grad_kernel_fn(g.M, g.K, g.minors);
     if (!fn) return fail(PQB_ERR_SIZE, "no adjoint kernel for this size");
-    static bool attr_set[11][4] = {};
-    if (!attr_set[g.M][g.K]) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    // dynamic LDS beyond the default: up to the CU's LDS minus the kernel's static LDS
+    static size_t dyn_limit[11][4][2] = {};
+    size_t& limit = dyn_limit[g.M][g.K][g.minors];
+    if (limit == 0) {
+        hipFuncAttributes fa;
+        hipError_t e = hipFuncGetAttributes(&fa, fn);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes(adjoint kernel)");
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, g_eng.device);
+        if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+        const size_t cu_lds = prop.maxSharedMemoryPerMultiProcessor;
+        if (cu_lds <= fa.sharedSizeBytes) return fail(PQB_ERR_DEVICE, "no LDS left for the adjoint kernel");
+        const size_t lim = cu_lds - fa.sharedSizeBytes;
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
         if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(adjoint kernel)");
-        attr_set[g.M][g.K] = true;
+        limit = lim;
     }
+    if (g.lds > limit) return fail(PQB_ERR_SIZE, "adjoint sweep needs more LDS than a compute unit has");
     int nb = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, g.lds);
     if (e != hipSuccess) return hip_fail(e, "occupancy query (adjoint kernel)");
@@ -84,14 +106,14 @@ static int grad_resident_waves_locked(const GradPlan& g, int64_t* waves) {
     return PQB_OK;
 }
 
-static int grad_plan_locked(int n, int64_t count, GradPlan* out) {
+static int grad_plan_locked(int n, int64_t count, GradPlan* out, bool minors = false) {
     int st = g_eng.ensure_init_locked(-1);
     if (st) return st;
-    const GradPlan shape = grad_shape(n);
+    const GradPlan shape = grad_shape(n, minors);
     int64_t wr = 0;
     st = grad_resident_waves_locked(shape, &wr);
     if (st) return st;
-    *out = make_grad_plan(n, count, wr);
+    *out = make_grad_plan(n, count, wr, minors);
     return PQB_OK;
 }
 
@@ -103,8 +125,9 @@ static int grad_check(int64_t count, int64_t n) {
     return PQB_OK;
 }
 
-// Launch on `st`: dA packed count x n x n, outputs packed.  `work` holds the
-// partials (grad_partials_bytes).
+// Launch on `st`: dA packed count x n x cols, outputs packed.  `work` holds the
+// partials (grad_partials_bytes).  Adjoint: d_grad count x n x n and d_perm
+// count; minors: d_grad count x (n+1), d_perm unused.
 static int grad_launch(const GradPlan& g, const double2* dA, int n, int64_t count, double2* d_perm,
                        double2* d_grad, void* work, hipStream_t st) {
     if (g.blocks > 0x7fffffff) return fail(PQB_ERR_SIZE, "adjoint launch too large");
@@ -112,17 +135,18 @@ static int grad_launch(const GradPlan& g, const double2* dA, int n, int64_t coun
     p.A = dA;
     p.partials = static_cast<double2*>(work);
     p.n = n;
+    p.cols = g.cols;
     p.L = g.L;
     p.H = g.H;
     p.wpt = (int)g.wpt;
-    hipError_t e = launch_grad(g.M, g.K, p, (int)g.blocks, g.lds, st);
+    hipError_t e = launch_grad(g.M, g.K, g.minors, p, (int)g.blocks, g.lds, st);
     if (e != hipSuccess) return hip_fail(e, "adjoint sweep launch");
     GradReduceParams r;
     r.partials = p.partials;
-    r.perm = d_perm;
-    r.grad = d_grad;
-    r.n = n;
-    r.wpt = (int)g.wpt;
+    r.out = d_grad;
+    r.tail = g.minors ? nullptr : d_perm;
+    r.main = g.minors ? n + 1 : n * n;
+    r.nblk = (int)(g.wpt / 4);
     r.count = (int)count;
     r.scale = std::ldexp(1.0, -(n - 1));
     e = launch_grad_reduce(r, st);
@@ -180,6 +204,49 @@ static int grad_host_entry(const double* A, int64_t count, int64_t n, int64_t ld
     if (e != hipSuccess) return hip_fail(e, "adjoint sweep");
     std::memcpy(perm_out, h, (size_t)count * 2 * sizeof(double));
     std::memcpy(grad_out, h + 2 * count, (size_t)count * mat * 2 * sizeof(double));
+    return PQB_OK;
+}
+
+// Column minors of one rows x (rows+1) matrix: out[j] = perm(A without column j).
+static int minors_host_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, double* out) {
+    if (!out) return fail(PQB_ERR_ARG, "output pointer is NULL");
+    if (rows < 0 || cols != rows + 1) return fail(PQB_ERR_ARG, "column minors need a rows x (rows + 1) matrix");
+    if (rows == 0) {  // the only minor is the empty matrix
+        out[0] = 1.0;
+        out[1] = 0.0;
+        return PQB_OK;
+    }
+    if (cols > PQB_MAX_COLS)
+        return fail(PQB_ERR_SIZE, "matrix has more than " + std::to_string(PQB_MAX_COLS) + " columns");
+    if (!A) return fail(PQB_ERR_ARG, "matrix pointer is NULL");
+    if (ld < cols) return fail(PQB_ERR_ARG, "leading dimension smaller than the number of columns");
+    const int R = (int)rows, C = (int)cols;
+
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    GradPlan g;
+    int st = grad_plan_locked(R, 1, &g, true);
+    if (st) return st;
+    const size_t a_bytes = (size_t)R * C * sizeof(double2);
+    const size_t out_bytes = (size_t)C * sizeof(double2);
+    const size_t part_bytes = grad_partials_bytes(R, 1, g.wpt, true);
+    const size_t a_off = kCounterBytes;  // offset 0 holds the plain sweep's grid counter
+    const size_t out_off = a_off + ((a_bytes + 255) / 256) * 256;
+    const size_t part_off = out_off + ((out_bytes + 255) / 256) * 256;
+    st = g_eng.ensure_buffers_locked(part_off + part_bytes, std::max(a_bytes, out_bytes));
+    if (st) return st;
+    double* h = static_cast<double*>(g_eng.hbuf);
+    for (int i = 0; i < R; ++i)
+        std::memcpy(h + 2 * (size_t)i * C, A + 2 * (size_t)i * ld, (size_t)C * 2 * sizeof(double));
+    char* d = static_cast<char*>(g_eng.dbuf);
+    hipError_t e = hipMemcpyAsync(d + a_off, h, a_bytes, hipMemcpyHostToDevice, g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+    st = grad_launch(g, reinterpret_cast<const double2*>(d + a_off), R, 1, nullptr,
+                     reinterpret_cast<double2*>(d + out_off), d + part_off, g_eng.stream);
+    if (st) return st;
+    e = hipMemcpyAsync(h, d + out_off, out_bytes, hipMemcpyDeviceToHost, g_eng.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    if (e != hipSuccess) return hip_fail(e, "column-minors sweep");
+    std::memcpy(out, h, out_bytes);
     return PQB_OK;
 }
 
@@ -254,6 +321,14 @@ int pqb_grad_plan(int64_t count, int64_t n, int64_t waves_resident, int64_t* pla
     plan[5] = g.blocks;
     plan[6] = (int64_t)g.lds;
     return PQB_OK;
+}
+
+int pqb_perm_minors(const double* A, int64_t rows, int64_t cols, int64_t ld, double* out) {
+    try {
+        return minors_host_entry(A, rows, cols, ld, out);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
 }
 
 }  // extern "C"

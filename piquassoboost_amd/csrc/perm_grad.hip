@@ -61,6 +61,9 @@ __device__ __forceinline__ double qperm(double v) {
 }
 constexpr int kQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
 constexpr int kQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
+// (The same permutation through ds_swizzle moves the 8 moves per step to the
+// LDS pipe but puts its latency on the step's critical path: measured 4% slower
+// on the C5 workload at 2 waves per SIMD.)
 
 // LDS accesses of one wave to locations written by other lanes of the same
 // wave: LDS executes a wave's instructions in order, the fences keep the
@@ -116,12 +119,17 @@ __device__ __forceinline__ void grad_term(const double (&cr)[M], const double (&
     }
 }
 
-template <int M, int K>
+// MINORS = false: the adjoint sweep of square n x n matrices (n = p.n = p.cols).
+// MINORS = true: for an R x (R+1) matrix (R = p.n, p.cols = R + 1), the sweep
+// over the R rows' sign vectors with the leave-one-out products only: the sum
+// of T over all lanes is, per column j, 2^(R-1) perm(A without column j).  No
+// row is differentiated, so flips update the column sums only.
+template <int M, int K, bool MINORS>
 __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
     extern __shared__ double2 lds[];
     constexpr int NP = 4 * M;   // padded column count
     constexpr int KA = K > 0 ? K : 1;
-    const int n = p.n, R = p.n, L = p.L, H = p.H;
+    const int n = p.n, R = p.n, L = p.L, H = p.H, NCOL = p.cols;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wid = tid / kWave;
@@ -131,16 +139,18 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
     const int task = (int)(gw / p.wpt);                 // wpt % 4 == 0: one task per block
     const int wi = (int)(gw - (int64_t)task * p.wpt);
 
-    double2* As = lds;  // n x NP, zero-padded columns
-    double2* Tx = lds + (size_t)n * NP + (size_t)wid * (16 + H) * NP;  // 16 x NP transpose scratch
-    double2* Zh = Tx + 16 * NP;                                         // H x NP high-row accumulators
+    constexpr int kZh = MINORS ? 0 : 1;
+    double2* As = lds;  // R x NP, zero-padded columns
+    double2* Tx = lds + (size_t)R * NP + (size_t)wid * (16 + kZh * H) * NP;  // 16 x NP transpose scratch
+    double2* Zh = Tx + 16 * NP;                                               // H x NP high-row accumulators
 
-    const double2* Ag = p.A + (size_t)task * n * n;
-    for (int e = tid; e < n * NP; e += blockDim.x) {
+    const double2* Ag = p.A + (size_t)task * R * NCOL;
+    for (int e = tid; e < R * NP; e += blockDim.x) {
         const int i = e / NP, j = e - i * NP;
-        As[e] = j < n ? Ag[(size_t)i * n + j] : make_double2(0.0, 0.0);
+        As[e] = j < NCOL ? Ag[(size_t)i * NCOL + j] : make_double2(0.0, 0.0);
     }
-    for (int e = lane; e < H * NP; e += kWave) Zh[e] = make_double2(0.0, 0.0);
+    if (!MINORS)
+        for (int e = lane; e < H * NP; e += kWave) Zh[e] = make_double2(0.0, 0.0);
     __syncthreads();
 
     const int64_t NB = (int64_t)1 << H;
@@ -177,12 +187,18 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        if (q * M + k >= n) {  // padding column: factor 1
+        if (q * M + k >= NCOL) {  // padding column: factor 1
             cr[k] = 1.0;
+            ci[k] = 0.0;
+        }
+        if (!lane_on) {  // unused delta lane (n <= 4): every product is zero
+            cr[k] = 0.0;
             ci[k] = 0.0;
         }
     }
     const double sl = lane_on ? ((negs & 1) ? -1.0 : 1.0) : 0.0;
+    // lane sign = s_time * (-1)^popcount(d): s_time from the time rows (wave-uniform)
+    const double s_time = (__popcll((unsigned long long)gam0) & 1) ? -1.0 : 1.0;
 
     // register rows' data (row n-1-b for bit b < K)
     double arr[KA][M], ari[KA][M];
@@ -221,29 +237,44 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
             // ---- high-row flip: time bit K + h, h = ctz(g); new delta = +1 iff bit h+1 of g
             const int h = __builtin_ctzll((unsigned long long)g);
             const double f = ((g >> (h + 1)) & 1) ? 2.0 : -2.0;  // 2 * new delta
+            // sum over the delta lanes of s_lane * T: T is written in the lane
+            // frame and the reducer applies (-1)^popcount(d) (static per j and
+            // the lane's half) and s_time (wave-uniform, folded into f)
+            if constexpr (!MINORS) {
             wave_lds_sync();
 #pragma unroll
-            for (int k = 0; k < M; ++k) Tx[d * NP + q * M + k] = make_double2(sl * tr[k], sl * ti[k]);
+            for (int k = 0; k < M; ++k) Tx[d * NP + q * M + k] = make_double2(tr[k], ti[k]);
             wave_lds_sync();
             double sr = 0.0, si = 0.0;
             if (col < NP) {
 #pragma unroll
                 for (int j = 0; j < kPer; ++j) {
                     const double2 v = Tx[(half * kPer + j) * NP + col];
-                    sr += v.x;
-                    si += v.y;
+                    if (__builtin_popcount(j) & 1) {
+                        sr -= v.x;
+                        si -= v.y;
+                    } else {
+                        sr += v.x;
+                        si += v.y;
+                    }
                 }
             }
-            if (kPairs) {
+            if (kPairs) {  // lanes of the second half hold d = 8 + j: one more lane bit
+                if (half) {
+                    sr = -sr;
+                    si = -si;
+                }
                 sr += qperm<kQuadXor1>(sr);
                 si += qperm<kQuadXor1>(si);
             }
             if (col < NP && half == 0) {
+                const double fs = f * s_time;
                 double2 z = Zh[h * NP + col];
-                z.x = fma(-f, sr, z.x);
-                z.y = fma(-f, si, z.y);
+                z.x = fma(-fs, sr, z.x);
+                z.y = fma(-fs, si, z.y);
                 Zh[h * NP + col] = z;
             }
+            }  // !MINORS
             const int row = R - 1 - (K + L + h);
 #pragma unroll
             for (int k = 0; k < M; ++k) {
@@ -264,8 +295,10 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
             const double f = up ? 2.0 : -2.0;
 #pragma unroll
             for (int k = 0; k < M; ++k) {
-                zr[b][k] = fma(-f, tr[k], zr[b][k]);
-                zi[b][k] = fma(-f, ti[k], zi[b][k]);
+                if constexpr (!MINORS) {
+                    zr[b][k] = fma(-f, tr[k], zr[b][k]);
+                    zi[b][k] = fma(-f, ti[k], zi[b][k]);
+                }
                 cr[k] = fma(f, arr[b][k], cr[k]);
                 ci[k] = fma(f, ari[b][k], ci[k]);
             }
@@ -307,6 +340,25 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
             li[b] += qperm<kQuadXor1>(li[b]);
         }
     }
+    if constexpr (MINORS) {
+        // per column j: sum over the block's lanes of T_j, in wave order
+        __syncthreads();
+        double2* acc = As;  // NCOL <= R x NP
+        for (int w = 0; w < 4; ++w) {
+            if (wid == w && col < NCOL && half == 0) {
+                double vr = tbr, vi = tbi;
+                if (w > 0) {
+                    vr += acc[col].x;
+                    vi += acc[col].y;
+                }
+                acc[col] = make_double2(vr, vi);
+            }
+            __syncthreads();
+        }
+        double2* out = p.partials + ((size_t)task * (p.wpt / 4) + wi / 4) * NCOL;
+        for (int e = tid; e < NCOL; e += blockDim.x) out[e] = acc[e];
+        return;
+    }
     double zbr[KA], zbi[KA];
 #pragma unroll
     for (int b = 0; b < KA; ++b) {
@@ -332,22 +384,6 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
         }
     }
 
-    const int E = n * n + 1;
-    double2* out = p.partials + ((size_t)task * p.wpt + wi) * E;
-    if (col < n && half == 0) {
-        out[col] = make_double2(tbr, tbi);  // row 0
-        for (int b = 0; b < L; ++b) out[(size_t)(R - 1 - (K + b)) * n + col] = make_double2(lr[b], li[b]);
-#pragma unroll
-        for (int b = 0; b < K; ++b) {
-            const double s = ((gam_end >> b) & 1) ? -1.0 : 1.0;
-            out[(size_t)(R - 1 - b) * n + col] = make_double2(fma(s, tbr, zbr[b]), fma(s, tbi, zbi[b]));
-        }
-        for (int h = 0; h < H; ++h) {
-            const double s = ((gam_end >> (K + h)) & 1) ? -1.0 : 1.0;
-            const double2 z = Zh[h * NP + col];
-            out[(size_t)(R - 1 - (K + L + h)) * n + col] = make_double2(fma(s, tbr, z.x), fma(s, tbi, z.y));
-        }
-    }
     // permanent: one lane of each quad carries the term sum
     double pr = (q == 0) ? sl * (er - orr) : 0.0;
     double pi = (q == 0) ? sl * (ei - oi) : 0.0;
@@ -356,26 +392,80 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
         pr += __shfl_xor(pr, m, kWave);
         pi += __shfl_xor(pi, m, kWave);
     }
-    if (lane == 0) out[(size_t)n * n] = make_double2(pr, pi);
+    __shared__ double2 perm_w[4];
+    if (lane == 0) perm_w[wid] = make_double2(pr, pi);
+
+    // ---- the block's four waves add their n x n partials in the staged
+    // matrix's place (no wave reads it any more), in wave order
+    __syncthreads();
+    double2* acc = As;  // n x n <= n x NP
+    for (int w = 0; w < 4; ++w) {
+        if (wid == w && col < n && half == 0) {
+            auto put = [&](int row, double vr, double vi) {
+                double2* a = acc + row * n + col;
+                if (w > 0) {
+                    const double2 o = *a;
+                    vr += o.x;
+                    vi += o.y;
+                }
+                *a = make_double2(vr, vi);
+            };
+            put(0, tbr, tbi);  // row 0
+            for (int b = 0; b < L; ++b) put(R - 1 - (K + b), lr[b], li[b]);
+#pragma unroll
+            for (int b = 0; b < K; ++b) {
+                const double s = ((gam_end >> b) & 1) ? -1.0 : 1.0;
+                put(R - 1 - b, fma(s, tbr, zbr[b]), fma(s, tbi, zbi[b]));
+            }
+            for (int h = 0; h < H; ++h) {
+                const double s = ((gam_end >> (K + h)) & 1) ? -1.0 : 1.0;
+                const double2 z = Zh[h * NP + col];
+                put(R - 1 - (K + L + h), fma(s, tbr, z.x), fma(s, tbi, z.y));
+            }
+        }
+        __syncthreads();
+    }
+    const int E = n * n + 1;
+    double2* out = p.partials + ((size_t)task * (p.wpt / 4) + wi / 4) * E;
+    for (int e = tid; e < n * n; e += blockDim.x) out[e] = acc[e];
+    if (tid == 0) {
+        double2 s = perm_w[0];
+        for (int w = 1; w < 4; ++w) {
+            s.x += perm_w[w].x;
+            s.y += perm_w[w].y;
+        }
+        out[(size_t)n * n] = s;
+    }
 }
 
-// Sum of the per-wave partials of each task (double-double), scaled.
-__global__ void grad_reduce_kernel(GradReduceParams p) {
-    const int E = p.n * p.n + 1;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)p.count * E) return;
-    const int task = (int)(idx / E);
-    const int e = (int)(idx - (int64_t)task * E);
-    const double2* src = p.partials + (size_t)task * p.wpt * E + e;
+// Sum of the per-block partials of each task (double-double, fixed order),
+// scaled.  Block = 64 elements x 4 slices of the partial index.
+__global__ __launch_bounds__(256) void grad_reduce_kernel(GradReduceParams p) {
+    __shared__ ddc red[4][64];
+    const int E = p.main + (p.tail ? 1 : 0);
+    const int chunks = (E + 63) / 64;
+    const int task = blockIdx.x / chunks;
+    const int el = threadIdx.x & 63;
+    const int slice = threadIdx.x >> 6;
+    const int e = (blockIdx.x - task * chunks) * 64 + el;
     ddc s = {0.0, 0.0, 0.0, 0.0};
-    for (int w = 0; w < p.wpt; ++w) {
-        const double2 v = src[(size_t)w * E];
-        ddc_add(s, ddc_from(v.x, v.y));
+    if (e < E) {
+        const double2* src = p.partials + (size_t)task * p.nblk * E + e;
+#pragma unroll 8
+        for (int b = slice; b < p.nblk; b += 4) {
+            const double2 v = src[(size_t)b * E];
+            ddc_add(s, ddc_from(v.x, v.y));
+        }
     }
-    s = ddc_scale(s, p.scale);
-    const double2 r = make_double2(s.hr + s.lr, s.hi + s.li);
-    if (e < E - 1) p.grad[(size_t)task * (E - 1) + e] = r;
-    else p.perm[task] = r;
+    red[slice][el] = s;
+    __syncthreads();
+    if (slice == 0 && e < E) {
+        for (int k = 1; k < 4; ++k) ddc_add(s, red[k][el]);
+        s = ddc_scale(s, p.scale);
+        const double2 r = make_double2(s.hr + s.lr, s.hi + s.li);
+        if (e < p.main) p.out[(size_t)task * p.main + e] = r;
+        else p.tail[task] = r;
+    }
 }
 
 // ------------------------------------------------------------ dispatch --
@@ -383,19 +473,25 @@ __global__ void grad_reduce_kernel(GradReduceParams p) {
 #define PQB_GRAD_CASES(X) \
     X(1, 0) X(2, 0) X(2, 1) X(2, 2) X(2, 3) X(3, 3) X(4, 3) X(5, 3) X(6, 2) X(7, 2) X(8, 1) X(9, 1) X(10, 1)
 
-const void* grad_kernel_fn(int M, int K) {
-#define PQB_CASE(MM, KK) \
-    if (M == MM && K == KK) return reinterpret_cast<const void*>(&grad_sweep_kernel<MM, KK>);
+const void* grad_kernel_fn(int M, int K, bool minors) {
+#define PQB_CASE(MM, KK)                                                                        \
+    if (M == MM && K == KK)                                                                     \
+        return minors ? reinterpret_cast<const void*>(&grad_sweep_kernel<MM, KK, true>)         \
+                      : reinterpret_cast<const void*>(&grad_sweep_kernel<MM, KK, false>);
     PQB_GRAD_CASES(PQB_CASE)
 #undef PQB_CASE
     return nullptr;
 }
 
-hipError_t launch_grad(int M, int K, const GradParams& p, int nblocks, size_t lds_bytes, hipStream_t st) {
-#define PQB_CASE(MM, KK)                                                                      \
-    if (M == MM && K == KK) {                                                                 \
-        hipLaunchKernelGGL((grad_sweep_kernel<MM, KK>), dim3(nblocks), dim3(256), lds_bytes, st, p); \
-        return hipGetLastError();                                                             \
+hipError_t launch_grad(int M, int K, bool minors, const GradParams& p, int nblocks, size_t lds_bytes,
+                       hipStream_t st) {
+#define PQB_CASE(MM, KK)                                                                                     \
+    if (M == MM && K == KK) {                                                                                \
+        if (minors)                                                                                          \
+            hipLaunchKernelGGL((grad_sweep_kernel<MM, KK, true>), dim3(nblocks), dim3(256), lds_bytes, st, p);  \
+        else                                                                                                 \
+            hipLaunchKernelGGL((grad_sweep_kernel<MM, KK, false>), dim3(nblocks), dim3(256), lds_bytes, st, p); \
+        return hipGetLastError();                                                                            \
     }
     PQB_GRAD_CASES(PQB_CASE)
 #undef PQB_CASE
@@ -403,10 +499,8 @@ hipError_t launch_grad(int M, int K, const GradParams& p, int nblocks, size_t ld
 }
 
 hipError_t launch_grad_reduce(const GradReduceParams& p, hipStream_t st) {
-    const int64_t total = (int64_t)p.count * (p.n * p.n + 1);
-    const int bd = 256;
-    const int64_t nb = (total + bd - 1) / bd;
-    hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)nb), dim3(bd), 0, st, p);
+    const int64_t chunks = (p.main + (p.tail ? 1 : 0) + 63) / 64;
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)(chunks * p.count)), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 

@@ -91,25 +91,30 @@ const void* rep_batch_kernel_fn(int range);
 hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks, hipStream_t st);
 
 // ---- adjoint sweep (permanent + all first derivatives of square matrices) ----
+// Adjoint mode: square n x n matrices.  Minors mode: n x (n+1) matrices, the
+// n+1 permanents with one column deleted.
 struct GradParams {
-    const double2* A;    // count x n x n, packed row-major
-    double2* partials;   // count x wpt x (n*n + 1): G row-major, then the permanent
-    int n;               // rows = columns
+    const double2* A;    // count x n x cols, packed row-major
+    double2* partials;   // count x (wpt / 4) x E, one per block.  Adjoint: E = n*n + 1 (G
+                         // row-major, then the permanent); minors: E = cols
+    int n;               // rows
+    int cols;            // columns: n (adjoint) or n + 1 (minors)
     int L;               // delta-lane bits (<= 4)
     int H;               // high bits: K + L + H = n - 1 (K is the kernel's template argument)
     int wpt;             // waves per task, multiple of 4
 };
 
 struct GradReduceParams {
-    const double2* partials;
-    double2* perm;       // count
-    double2* grad;       // count x n x n
-    int n, wpt, count;
+    const double2* partials;  // count x nblk x (main + (tail ? 1 : 0))
+    double2* out;        // count x main
+    double2* tail;       // count, or NULL
+    int main, nblk, count;  // nblk: partials (blocks) per task
     double scale;        // 2^-(n-1)
 };
 
-const void* grad_kernel_fn(int M, int K);
-hipError_t launch_grad(int M, int K, const GradParams& p, int nblocks, size_t lds_bytes, hipStream_t st);
+const void* grad_kernel_fn(int M, int K, bool minors);
+hipError_t launch_grad(int M, int K, bool minors, const GradParams& p, int nblocks, size_t lds_bytes,
+                       hipStream_t st);
 hipError_t launch_grad_reduce(const GradReduceParams& p, hipStream_t st);
 
 hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, int ntasks, hipStream_t st);

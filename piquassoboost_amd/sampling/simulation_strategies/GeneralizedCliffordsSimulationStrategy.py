@@ -15,8 +15,10 @@ Mirrors piquassoboost.sampling.simulation_strategies.GeneralizedCliffordsSimulat
 
 compute_pmf needs, for every occupied input mode, the permanent of the effective matrix
 with one photon removed from that mode (the reference calls
-BBFGPermanentCalculatorRepeated once per mode); here all of them go to the engine as one
-batched launch.  The random numbers come from the shared std::mt19937 restated in
+BBFGPermanentCalculatorRepeated once per mode).  These are the column minors of the
+expanded (k-1) x k matrix, and one column-minors sweep on the GPU (pqb_perm_minors) gives
+all of them; the per-mode batched form is kept as method="batch".  The random numbers
+come from the shared std::mt19937 restated in
 ``piquassoboost_amd.sampling.mt19937`` (seeded as the reference does), so seeded runs give
 the reference's samples.
 """
@@ -45,20 +47,9 @@ def modes_state_to_particle_state(mode_state) -> np.ndarray:
     return np.repeat(np.arange(s.shape[0], dtype=np.int64), s)
 
 
-def compute_pmf(interferometer_matrix, sample, current_input, permanent_batch=None) -> np.ndarray:
-    """Probability of each output mode receiving the next photon
-    (samplingHelperFunctions.cpp:78-342).
-
-    For each occupied input mode idx: permanent_addends[idx] = perm of the effective matrix
-    (rows: occupied output modes of `sample`, columns: occupied input modes after removing
-    one photon from idx, with their multiplicities).  Then
-    pmf[mdx] = |sum_idx current_input[idx] * addend[idx] * U[mdx, idx]|^2, normalised.
-    `permanent_batch` (default: the GPU engine) takes (matrix, row_mult, col_mult) tasks."""
-    U = np.asarray(interferometer_matrix)
-    sample = np.asarray(sample, dtype=np.int64)
-    current_input = np.asarray(current_input, dtype=np.int64)
-    perm_fn = C.permanent_batch if permanent_batch is None else permanent_batch
-    col_idx = np.flatnonzero(current_input > 0)
+def _addends_batched(U, sample, current_input, col_idx, perm_fn):
+    """One repeated-row permanent per occupied input mode, as the reference computes them
+    (samplingHelperFunctions.cpp:137-172), submitted as one batch."""
     out_rows = np.flatnonzero(sample > 0)
     tasks = []
     for idx in col_idx:
@@ -67,7 +58,46 @@ def compute_pmf(interferometer_matrix, sample, current_input, permanent_batch=No
         in_cols = np.flatnonzero(inp > 0)
         A = np.ascontiguousarray(U[np.ix_(out_rows, in_cols)], dtype=np.complex128)
         tasks.append((A, sample[out_rows], inp[in_cols]))
-    addends = np.asarray(perm_fn(tasks), dtype=np.complex128) if tasks else np.zeros(0, np.complex128)
+    return np.asarray(perm_fn(tasks), dtype=np.complex128) if tasks else np.zeros(0, np.complex128)
+
+
+def _addends_minors(U, sample, current_input, col_idx):
+    """All addends from one column-minors sweep: the permanent with one photon removed from
+    input mode idx is the minor of the expanded (k-1) x k matrix (rows: sample photons,
+    columns: input photons) without one column of mode idx."""
+    rows = np.repeat(np.arange(sample.shape[0]), sample)
+    cols = np.repeat(np.arange(current_input.shape[0]), current_input)
+    minors = C.permanent_minors(np.ascontiguousarray(U[np.ix_(rows, cols)], dtype=np.complex128))
+    first = {int(mode): j for j, mode in reversed(list(enumerate(cols)))}
+    return np.array([minors[first[int(idx)]] for idx in col_idx], dtype=np.complex128)
+
+
+def compute_pmf(interferometer_matrix, sample, current_input, permanent_batch=None,
+                method: str = "minors") -> np.ndarray:
+    """Probability of each output mode receiving the next photon
+    (samplingHelperFunctions.cpp:78-342).
+
+    For each occupied input mode idx: permanent_addends[idx] = perm of the effective matrix
+    (rows: occupied output modes of `sample`, columns: occupied input modes after removing
+    one photon from idx, with their multiplicities).  Then
+    pmf[mdx] = |sum_idx current_input[idx] * addend[idx] * U[mdx, idx]|^2, normalised.
+
+    method "minors" (default): all addends from one column-minors sweep on the GPU;
+    "batch": one repeated-row permanent per mode in one batched launch, through
+    `permanent_batch` when given (a callable taking (matrix, row_mult, col_mult) tasks)."""
+    U = np.asarray(interferometer_matrix)
+    sample = np.asarray(sample, dtype=np.int64)
+    current_input = np.asarray(current_input, dtype=np.int64)
+    col_idx = np.flatnonzero(current_input > 0)
+    if int(sample.sum()) + 1 != int(current_input.sum()):
+        raise ValueError("compute_pmf: the input must hold one photon more than the sample")
+    if method == "minors" and permanent_batch is None:
+        addends = _addends_minors(U, sample, current_input, col_idx)
+    elif method in ("minors", "batch"):
+        perm_fn = C.permanent_batch if permanent_batch is None else permanent_batch
+        addends = _addends_batched(U, sample, current_input, col_idx, perm_fn)
+    else:
+        raise ValueError(f"unknown compute_pmf method {method!r}")
     perms = np.zeros(sample.shape[0], dtype=np.complex128)
     for k, idx in enumerate(col_idx):  # the reference's order of accumulation
         perms = perms + (current_input[idx] * addends[k]) * U[:, idx]
@@ -106,10 +136,12 @@ class GeneralizedCliffordsBSimulationStrategy:
     """GeneralizedCliffordsSimulationStrategy.py:33-47 /
     CGeneralizedCliffordsBSimulationStrategy.cpp (lossless)."""
 
-    def __init__(self, interferometer_matrix, seed=None, lib: int = 0, permanent_batch=None):
+    def __init__(self, interferometer_matrix, seed=None, lib: int = 0, permanent_batch=None,
+                 pmf_method: str = "minors"):
         self.interferometer_matrix = np.ascontiguousarray(interferometer_matrix, dtype=np.complex128)
         self.lib = lib
         self._perm = permanent_batch
+        self._pmf_method = pmf_method
         seed_random_generator(int(time.time()))  # the C++ constructor seeds with time(NULL)
         if seed is not None:
             self.seed(seed)
@@ -120,7 +152,7 @@ class GeneralizedCliffordsBSimulationStrategy:
     def _fill_r_sample(self, sample, current_input, working, photons) -> None:
         while photons > int(sample.sum()):
             working = update_input_by_single_photon(current_input, working)
-            pmf = compute_pmf(self.interferometer_matrix, sample, current_input, self._perm)
+            pmf = compute_pmf(self.interferometer_matrix, sample, current_input, self._perm, self._pmf_method)
             sample_from_pmf(sample, pmf)
 
     def _photons(self, input_state) -> int:
@@ -144,9 +176,11 @@ class GeneralizedCliffordsBUniformLossesSimulationStrategy(GeneralizedCliffordsB
     CGeneralizedCliffordsBUniformLossesSimulationStrategy.cpp: transmissivity t, each photon
     survives with probability t^2."""
 
-    def __init__(self, interferometer_matrix, transmissivity, seed=None, lib: int = 0, permanent_batch=None):
+    def __init__(self, interferometer_matrix, transmissivity, seed=None, lib: int = 0, permanent_batch=None,
+                 pmf_method: str = "minors"):
         self.transmissivity = float(transmissivity)
-        super().__init__(interferometer_matrix, seed=seed, lib=lib, permanent_batch=permanent_batch)
+        super().__init__(interferometer_matrix, seed=seed, lib=lib, permanent_batch=permanent_batch,
+                         pmf_method=pmf_method)
 
     def _photons(self, input_state) -> int:
         """calculate_current_photon_number (:285-296) over the binomial weights (:298-316)."""

@@ -214,3 +214,26 @@ def test_grad_argument_errors_without_device():
         C.permanent_grad(np.ones((41, 41)))
     with pytest.raises(C.EngineError):
         C.grad_plan(0, 5, 1024)
+
+
+def test_minors_edge_and_shape_errors_without_device():
+    assert C.permanent_minors(np.zeros((0, 1))).tolist() == [1.0]
+    with pytest.raises(C.EngineError, match="rows x \\(rows \\+ 1\\)"):
+        C.permanent_minors(np.ones((3, 3)))
+    with pytest.raises(C.EngineError):
+        C.permanent_minors(np.ones((40, 41)))  # 41 columns
+
+
+def test_batched_front_end_errors_without_device():
+    from piquassoboost_amd.sampling.BatchedPermanentCalculator import BatchednPermanentCalculator
+
+    with pytest.raises(Exception, match="interferometer matrix is zero"):
+        BatchednPermanentCalculator().calculate()
+    b = BatchednPermanentCalculator(np.eye(3))
+    assert b.calculate().shape == (1, 0)  # nothing accumulated
+    b.reserve_space(1)
+    b.add(np.array([1, 1, 0]), np.array([0, 1, 1]), 0)
+    with pytest.raises(Exception, match="repeated DFE not implemented yet"):
+        b.calculate(lib=2)
+    with pytest.raises(Exception, match="Wrong value set for permanent library"):
+        b.calculate(lib=5)

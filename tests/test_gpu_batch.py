@@ -65,3 +65,21 @@ def test_c4_replay_deterministic(c4):
     res2 = C.permanent_batch(tasks)
     assert res2 == res
     assert C.batch_replay_ms(2) > 0.0
+
+
+def test_batched_front_end(c4):
+    """BatchednPermanentCalculator (one interferometer, accumulated state pairs) on 64 of
+    the uneven transitions: both lib codes against the per-call path and the oracle."""
+    from piquassoboost_amd.sampling.BatchedPermanentCalculator import BatchednPermanentCalculator
+
+    U, pairs, tasks, res = c4
+    b = BatchednPermanentCalculator(U)
+    b.reserve_space(64)
+    for k in range(64):
+        b.add(pairs[k][0], pairs[k][1], k)
+    for lib in (0, 1):
+        out = b.calculate(lib)
+        assert out.shape == (1, 64)
+        for k in range(0, 64, 9):
+            A, rows, cols = tasks[k]
+            assert rel_err(out[0, k], oracle.bbfg_repeated(A, cols, rows, "dd")) < 1e-10, (lib, k)

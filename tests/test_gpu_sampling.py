@@ -57,13 +57,36 @@ def test_uniform_losses_seeded(seeding):
 
 
 def test_compute_pmf_large_step():
-    """14 photons in 40 modes, collisions in input and sample."""
+    """14 photons in 40 modes, collisions in input and sample: the column-minors sweep,
+    the batched per-mode permanents and the oracle agree."""
     U = haar_unitary(40, 77)
     rng = np.random.default_rng(78)
     current_input = np.zeros(40, np.int64)
     np.add.at(current_input, rng.choice(40, size=14), 1)
     sample = np.zeros(40, np.int64)
     np.add.at(sample, rng.choice(40, size=13), 1)
-    got = compute_pmf(U, sample, current_input)
     ref = compute_pmf(U, sample, current_input, permanent_batch=oracle_batch)
-    assert np.abs(got - ref).max() <= 1e-10 * ref.max()
+    for method in ("minors", "batch"):
+        got = compute_pmf(U, sample, current_input, method=method)
+        assert np.abs(got - ref).max() <= 1e-10 * ref.max(), method
+
+
+@pytest.mark.parametrize("R", [1, 2, 3, 4, 5, 7, 8, 12, 20, 27])
+def test_column_minors_against_oracle(R):
+    """pqb_perm_minors on R x (R+1) matrices (every lane and register-bit split up to R=8)."""
+    from conftest import random_complex
+    from piquassoboost_amd import _capi as C
+
+    rng = np.random.default_rng(500 + R)
+    A = random_complex(rng, R, R + 1)
+    got = C.permanent_minors(A)
+    ref = np.array([oracle.bbfg(np.delete(A, j, 1), "dd") for j in range(R + 1)])
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_column_minors_shapes():
+    from piquassoboost_amd import _capi as C
+
+    assert C.permanent_minors(np.zeros((0, 1))).tolist() == [1.0]
+    with pytest.raises(C.EngineError):
+        C.permanent_minors(np.ones((3, 3)))
