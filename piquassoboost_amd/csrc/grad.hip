@@ -70,7 +70,7 @@ static GradPlan make_grad_plan(int n, int64_t count, int64_t waves_resident, boo
 }
 
 // One partial per block of 4 waves: n x n gradient + permanent, or the n+1 column minors.
-static size_t grad_partial_elems(int n, bool minors) { return minors ? (size_t)n + 1 : (size_t)n * n + 1; }
+static size_t grad_partial_elems(int n, bool minors) { return minors ? (size_t)n + 1 : (size_t)n * n; }
 
 static size_t grad_partials_bytes(int n, int64_t count, int64_t wpt, bool minors = false) {
     return (size_t)count * (size_t)(wpt / 4) * grad_partial_elems(n, minors) * sizeof(double2);
@@ -144,13 +144,17 @@ static int grad_launch(const GradPlan& g, const double2* dA, int n, int64_t coun
     GradReduceParams r;
     r.partials = p.partials;
     r.out = d_grad;
-    r.tail = g.minors ? nullptr : d_perm;
+    r.tail = nullptr;
     r.main = g.minors ? n + 1 : n * n;
     r.nblk = (int)(g.wpt / 4);
     r.count = (int)count;
     r.scale = std::ldexp(1.0, -(n - 1));
     e = launch_grad_reduce(r, st);
     if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
+    if (!g.minors) {  // the permanent from row 0 of the gradient (Laplace expansion)
+        e = launch_grad_perm(dA, d_grad, d_perm, n, (int)count, st);
+        if (e != hipSuccess) return hip_fail(e, "permanent (Laplace row 0) launch");
+    }
     return PQB_OK;
 }
 

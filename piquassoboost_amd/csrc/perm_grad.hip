@@ -75,10 +75,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // One Gray step's contribution: y_j = (-1)^t * prod_{l != j} c_l for the
-// lane's M columns, accumulated into T; the full product into (er, ei).
+// lane's M columns, accumulated into T.  (The permanent itself is not
+// accumulated: it is sum_j a_0j G_0j of the result, grad_perm_kernel.)
 template <int M, bool ODD>
 __device__ __forceinline__ void grad_term(const double (&cr)[M], const double (&ci)[M], double (&tr)[M],
-                                          double (&ti)[M], double& er, double& ei) {
+                                          double (&ti)[M]) {
     // suffix products s_k = c_k ... c_{M-1}; s_0 is this quarter's product
     double sr[M], si[M];
     sr[M - 1] = cr[M - 1];
@@ -93,10 +94,6 @@ __device__ __forceinline__ void grad_term(const double (&cr)[M], const double (&
     const double g1r = sr[0] * x1r - si[0] * x1i;
     const double g1i = sr[0] * x1i + si[0] * x1r;
     const double x2r = qperm<kQuadXor2>(g1r), x2i = qperm<kQuadXor2>(g1i);
-    er = fma(g1r, x2r, er);
-    er = fma(-g1i, x2i, er);
-    ei = fma(g1r, x2i, ei);
-    ei = fma(g1i, x2r, ei);
     // prefix starts with the other three quarters' product, signed by (-1)^t
     double pr = x1r * x2r - x1i * x2i;
     double pi = x1r * x2i + x1i * x2r;
@@ -224,7 +221,6 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
             zi[b][k] = 0.0;
         }
     }
-    double er = 0.0, ei = 0.0, orr = 0.0, oi = 0.0;  // even / odd step products
 
     // reduction lanes: column `col` summed by one lane (NP > 32) or a lane pair
     constexpr bool kPairs = NP <= 32;
@@ -285,8 +281,8 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
         }
         // step sign (-1)^(tau - tau0): blocks of 2^K >= 2 steps start even;
         // with K = 0 every block is one step and the parity alternates
-        if (K == 0 && ((g - g0) & 1)) grad_term<M, true>(cr, ci, tr, ti, orr, oi);
-        else grad_term<M, false>(cr, ci, tr, ti, er, ei);
+        if (K == 0 && ((g - g0) & 1)) grad_term<M, true>(cr, ci, tr, ti);
+        else grad_term<M, false>(cr, ci, tr, ti);
 #pragma unroll
         for (int t = 1; t < (1 << K); ++t) {
             // ---- register-row flip: bit b = ctz(t); new delta = +1 iff bit b+1 of tau
@@ -302,8 +298,8 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
                 cr[k] = fma(f, arr[b][k], cr[k]);
                 ci[k] = fma(f, ari[b][k], ci[k]);
             }
-            if (t & 1) grad_term<M, true>(cr, ci, tr, ti, orr, oi);
-            else grad_term<M, false>(cr, ci, tr, ti, er, ei);
+            if (t & 1) grad_term<M, true>(cr, ci, tr, ti);
+            else grad_term<M, false>(cr, ci, tr, ti);
         }
     }
 
@@ -384,17 +380,6 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
         }
     }
 
-    // permanent: one lane of each quad carries the term sum
-    double pr = (q == 0) ? sl * (er - orr) : 0.0;
-    double pi = (q == 0) ? sl * (ei - oi) : 0.0;
-#pragma unroll
-    for (int m = kWave / 2; m >= 1; m >>= 1) {
-        pr += __shfl_xor(pr, m, kWave);
-        pi += __shfl_xor(pi, m, kWave);
-    }
-    __shared__ double2 perm_w[4];
-    if (lane == 0) perm_w[wid] = make_double2(pr, pi);
-
     // ---- the block's four waves add their n x n partials in the staged
     // matrix's place (no wave reads it any more), in wave order
     __syncthreads();
@@ -425,17 +410,33 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
         }
         __syncthreads();
     }
-    const int E = n * n + 1;
-    double2* out = p.partials + ((size_t)task * (p.wpt / 4) + wi / 4) * E;
+    double2* out = p.partials + ((size_t)task * (p.wpt / 4) + wi / 4) * (n * n);
     for (int e = tid; e < n * n; e += blockDim.x) out[e] = acc[e];
-    if (tid == 0) {
-        double2 s = perm_w[0];
-        for (int w = 1; w < 4; ++w) {
-            s.x += perm_w[w].x;
-            s.y += perm_w[w].y;
-        }
-        out[(size_t)n * n] = s;
+}
+
+// perm(A) = sum_j a_0j G_0j (Laplace expansion along row 0 of the finished
+// gradient), products split exactly (FMA) and summed in double-double.  One
+// thread per matrix.
+__global__ void grad_perm_kernel(const double2* __restrict__ A, const double2* __restrict__ G,
+                                 double2* __restrict__ perm, int n, int count) {
+    const int task = blockIdx.x * blockDim.x + threadIdx.x;
+    if (task >= count) return;
+    const double2* a = A + (size_t)task * n * n;  // row 0
+    const double2* g = G + (size_t)task * n * n;  // row 0
+    double hr = 0.0, lr = 0.0, hi = 0.0, li = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double2 x = a[j], y = g[j];
+        // re = x.x y.x - x.y y.y, im = x.x y.y + x.y y.x, each product as p + e exactly
+        const double p1 = x.x * y.x, e1 = fma(x.x, y.x, -p1);
+        const double p2 = -x.y * y.y, e2 = fma(-x.y, y.y, -p2);
+        const double p3 = x.x * y.y, e3 = fma(x.x, y.y, -p3);
+        const double p4 = x.y * y.x, e4 = fma(x.y, y.x, -p4);
+        dd_add(hr, lr, p1, e1);
+        dd_add(hr, lr, p2, e2);
+        dd_add(hi, li, p3, e3);
+        dd_add(hi, li, p4, e4);
     }
+    perm[task] = make_double2(hr + lr, hi + li);
 }
 
 // Sum of the per-block partials of each task (double-double, fixed order),
@@ -501,6 +502,12 @@ hipError_t launch_grad(int M, int K, bool minors, const GradParams& p, int nbloc
 hipError_t launch_grad_reduce(const GradReduceParams& p, hipStream_t st) {
     const int64_t chunks = (p.main + (p.tail ? 1 : 0) + 63) / 64;
     hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)(chunks * p.count)), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_perm(const double2* A, const double2* G, double2* perm, int n, int count, hipStream_t st) {
+    const int bd = 64;
+    hipLaunchKernelGGL(grad_perm_kernel, dim3((count + bd - 1) / bd), dim3(bd), 0, st, A, G, perm, n, count);
     return hipGetLastError();
 }
 

@@ -95,8 +95,8 @@ hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks,
 // n+1 permanents with one column deleted.
 struct GradParams {
     const double2* A;    // count x n x cols, packed row-major
-    double2* partials;   // count x (wpt / 4) x E, one per block.  Adjoint: E = n*n + 1 (G
-                         // row-major, then the permanent); minors: E = cols
+    double2* partials;   // count x (wpt / 4) x E, one per block.  Adjoint: E = n*n (G
+                         // row-major); minors: E = cols
     int n;               // rows
     int cols;            // columns: n (adjoint) or n + 1 (minors)
     int L;               // delta-lane bits (<= 4)
@@ -116,6 +116,8 @@ const void* grad_kernel_fn(int M, int K, bool minors);
 hipError_t launch_grad(int M, int K, bool minors, const GradParams& p, int nblocks, size_t lds_bytes,
                        hipStream_t st);
 hipError_t launch_grad_reduce(const GradReduceParams& p, hipStream_t st);
+// perm = sum_j a_0j G_0j per matrix (A, G packed count x n x n)
+hipError_t launch_grad_perm(const double2* A, const double2* G, double2* perm, int n, int count, hipStream_t st);
 
 hipError_t launch_plain(int nc, const PlainParams& p, int nblocks, int bdim, int ntasks, hipStream_t st);
 hipError_t launch_plain_prep(const double2* A, double2* M2, int n, int64_t count, hipStream_t st);
