@@ -28,6 +28,10 @@ N = 30
 M_MODES = 60
 SEED = 1002
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
+# HBM bytes per launch of the n=30 sweep: FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a
+# 256 MiB copy kernel), measured with rocprofv3 --pmc on this workload
+# (profiles/plain_sweep_n30_counters.md).
+TRAFFIC_BYTES_N30 = 691098
 
 
 def flops_plain(n: int) -> float:
@@ -338,7 +342,7 @@ def main() -> int:
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_N30,
         },
         "event_ms_per_perm": ms_per_step,
         "perm": [perm.real, perm.imag],
