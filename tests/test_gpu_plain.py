@@ -179,3 +179,25 @@ def test_repeated_list_batched():
     for i in range(2):
         for j, o in enumerate(outs[i]):
             assert rel_err(res[i][j], oracle.bbfg_repeated(U, ins[i], o, "dd")) < 1e-12
+
+
+def test_concurrent_host_calls_are_serialised_and_deterministic():
+    """The host entry points hold one engine lock (ctypes releases the GIL around them):
+    calls from 8 Python threads give bit-identical results to sequential calls."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(99)
+    mats = [random_complex(rng, 14, 14) for _ in range(12)]
+
+    def work(k):
+        A = mats[k]
+        if k % 3 == 0:
+            return C.permanent(A)
+        if k % 3 == 1:
+            return C.permanent_repeated(A[:5, :5], np.array([1, 2, 1, 1, 2]), np.array([2, 1, 1, 2, 1]))
+        return complex(C.permanent_grad(A[:9, :9])[1][2, 3])
+
+    seq = [work(k) for k in range(len(mats))]
+    with ThreadPoolExecutor(8) as ex:
+        par = list(ex.map(work, range(len(mats))))
+    assert par == seq
