@@ -53,19 +53,30 @@ extern "C" {
  *                   Integer-valued real matrices whose permanent is provably
  *                   representable go through the exact integer sweep instead,
  *                   so the result is the correctly rounded exact permanent.
- * PQB_PREC_EXTENDED Reference "long double" / MPFR codes (lib 0, 1, 8, 9).
- *                   Same kernels as PQB_PREC_DOUBLE (the error of the FP64
- *                   sweep with dd reduction is reported per result by the
- *                   tests); kept as a separate code so callers keep their
- *                   intent and a wider-accumulator kernel can slot in.
+ * PQB_PREC_EXTENDED Reference "long double" codes (lib 0, 8, 11; repeated
+ *                   lib 1, 8).  Same kernels as PQB_PREC_DOUBLE (the error of
+ *                   the FP64 sweep with dd reduction is reported per result
+ *                   by the tests); kept as a separate code so callers keep
+ *                   their intent.
  * PQB_PREC_FP64     Force the FP64 sweep, never the integer route.
  * PQB_PREC_EXACT    Integer sweep modulo 2^128; fails with PQB_ERR_EXACT
  *                   unless every entry is an integer with zero imaginary part
- *                   and pqb_exact_bound() proves 2^(rows-1) |perm| < 2^126.5. */
+ *                   and pqb_exact_bound() proves 2^(rows-1) |perm| < 2^126.5.
+ * PQB_PREC_INF      Double-double sweep: every column sum, product and
+ *                   partial sum carries a 106-bit significand and the result
+ *                   is rounded once.  Replaces the reference's exact MPFR
+ *                   calculators (use_inf: BBFGPermanentCalculator.cpp:100-107,
+ *                   BBFGPermanentCalculatorRepeated.cpp:115-122; lib 1 plain,
+ *                   lib 9 repeated), whose result is the exact sum rounded to
+ *                   nearest: the two agree unless the sum cancels by more than
+ *                   ~2^45.  Integer-valued real matrices that PQB_PREC_EXACT
+ *                   accepts take the exact integer sweep (plain form).  About
+ *                   10x the time of the FP64 sweep. */
 #define PQB_PREC_DOUBLE 0
 #define PQB_PREC_EXTENDED 1
 #define PQB_PREC_FP64 2
 #define PQB_PREC_EXACT 3
+#define PQB_PREC_INF 4
 
 /* Largest number of columns (distinct column slots for the repeated form)
  * the kernels are instantiated for. */

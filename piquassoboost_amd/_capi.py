@@ -30,6 +30,7 @@ PREC_DOUBLE = 0
 PREC_EXTENDED = 1
 PREC_FP64 = 2
 PREC_EXACT = 3
+PREC_INF = 4  # double-double sweep (the reference's MPFR "Inf" calculators)
 
 MAX_COLS = 40
 

@@ -40,10 +40,16 @@ CFLAGS = [
     f"-I{CSRC}",
 ]
 
+# Per-source flags.  The double-double sweep is built from error-free
+# transformations, which stay error-free only if no product is fused into a
+# later sum: no contraction anywhere in that translation unit (a pragma does
+# not cover the negations and the math-header wrappers that get inlined).
+EXTRA_FLAGS = {"perm_dd.hip": ["-ffp-contract=off"]}
+
 
 def _compile(src: Path, verbose: bool) -> Path:
     obj = OBJ / (src.stem + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *CFLAGS, *EXTRA_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -60,7 +60,7 @@ static long double sweep_steps(const std::vector<int64_t>& v) {
 }
 
 int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* rmult,
-              const int64_t* cmult, uint64_t target_L, TaskPlan& P) {
+              const int64_t* cmult, uint64_t target_L, TaskPlan& P, bool plain) {
     P = TaskPlan();
     std::vector<int64_t> ri, rm, ci, cm;
     int64_t N = 0, Nc = 0;
@@ -76,13 +76,17 @@ int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int
             cm.push_back(cmult[j]);
             Nc += cmult[j];
         }
-    if (N != Nc) return fail(PQB_ERR_STATE, "Number of input and output states should be equal");
+    // plain: a rows x cols matrix with every multiplicity 1 and rows <= cols + 1
+    // (the BBFG sum of BBFGPermanentCalculator.hpp:87-235, which is not
+    // transpose invariant when rows != cols)
+    if (!plain && N != Nc) return fail(PQB_ERR_STATE, "Number of input and output states should be equal");
     if (N == 0) {
         P.trivial = true;
         return PQB_OK;
     }
-    if (N > kMaxSlots) return fail(PQB_ERR_SIZE, "more than " + std::to_string(kMaxSlots) + " photons");
-    const bool transpose = sweep_steps(cm) < sweep_steps(rm);
+    if (Nc > kMaxSlots || (!plain && N > kMaxSlots))
+        return fail(PQB_ERR_SIZE, "more than " + std::to_string(kMaxSlots) + " photons");
+    const bool transpose = !plain && sweep_steps(cm) < sweep_steps(rm);
     // element (r, c) of the oriented compact matrix
     auto elem = [&](int64_t r, int64_t c, double* re, double* im) {
         // oriented row r / column c -> original row i, column j
@@ -215,8 +219,8 @@ struct BatchBuffers {
     size_t cap_tasks = 0, cap_items = 0, cap_data = 0;
     size_t off_binom = 0, off_tasks = 0, off_items = 0, off_order = 0, off_data = 0, off_part = 0, off_out = 0,
            total = 0;
-    hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    int grid[5] = {0, 0, 0, 0, 0};
+    hipGraphExec_t graph[kRanges] = {};  // FP64 slot ranges, then the double-double ones
+    int grid[kRanges] = {};
     // last prepared batch (for replays)
     int last_range = -1;
     int last_ntasks = 0;
@@ -296,7 +300,8 @@ static int batch_grid_locked(int range) {
     BatchBuffers& B = g_batch;
     if (B.grid[range] == 0) {
         int nb = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rep_batch_kernel_fn(range), 256, 0);
+        const void* fn = range >= kDDRange0 ? rep_dd_kernel_fn(range) : rep_batch_kernel_fn(range);
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0);
         if (e != hipSuccess || nb <= 0) nb = 1;
         B.grid[range] = nb * g_eng.cus;
     }
@@ -311,7 +316,8 @@ static int batch_graph_locked(int range, hipGraphExec_t* out) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamBeginCapture(g_eng.stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-        hipError_t le = launch_rep_batch(range, p, grid, g_eng.stream);
+        hipError_t le = range >= kDDRange0 ? launch_rep_dd(range, p, grid, g_eng.stream)
+                                           : launch_rep_batch(range, p, grid, g_eng.stream);
         e = hipStreamEndCapture(g_eng.stream, &g);
         if (le != hipSuccess) return hip_fail(le, "batch launch (capture)");
         if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -339,8 +345,9 @@ static uint64_t choose_L(long double S, long double total, int64_t lanes) {
     return (uint64_t)L;
 }
 
-// Run a list of planned tasks; results (6 doubles each) into out6.
-static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& out6) {
+// Run a list of planned tasks; results (6 doubles each) into out6.  dd: the
+// double-double sweep (a chunk per quad of lanes) instead of the FP64 one.
+static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& out6, bool dd) {
     BatchBuffers& B = g_batch;
     const size_t nt = plans.size();
     out6.assign(nt * 6, 0.0);
@@ -360,7 +367,7 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
         max_ns = std::max(max_ns, plans[i].NS);
     }
     if (live.empty()) return PQB_OK;
-    int range = rep_range_for(min_ns, max_ns);
+    const int range = dd ? rep_dd_range_for(max_ns) : rep_range_for(min_ns, max_ns);
     if (range < 0) {
         // split at 32 slots and run the two halves
         std::vector<TaskPlan> lo, hi;
@@ -375,9 +382,9 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
             }
         }
         std::vector<double> olo, ohi;
-        int st = run_tasks_locked(lo, olo);
+        int st = run_tasks_locked(lo, olo, dd);
         if (st) return st;
-        st = run_tasks_locked(hi, ohi);
+        st = run_tasks_locked(hi, ohi, dd);
         if (st) return st;
         for (size_t k = 0; k < ilo.size(); ++k) std::copy(&olo[6 * k], &olo[6 * k] + 6, &out6[6 * ilo[k]]);
         for (size_t k = 0; k < ihi.size(); ++k) std::copy(&ohi[6 * k], &ohi[6 * k] + 6, &out6[6 * ihi[k]]);
@@ -389,15 +396,16 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     std::vector<RepItem> items;
     std::vector<ItemCost> costs;
     std::vector<std::pair<uint32_t, uint32_t>> task_items(live.size());
-    const uint64_t lane_steps = 2048;  // target Gray steps per lane per item
+    const uint64_t lane_steps = 2048;     // target Gray steps per lane (quad) per item
+    const uint64_t walkers = dd ? 16 : 64;  // chunks a wave works on at once
     for (size_t k = 0; k < live.size(); ++k) {
         const TaskPlan& P = plans[live[k]];
         const uint64_t per_lane = std::max<uint64_t>(1, lane_steps / P.L);
-        const uint64_t per_item = 64 * per_lane;
+        const uint64_t per_item = walkers * per_lane;
         task_items[k].first = (uint32_t)items.size();
         for (uint64_t c0 = 0; c0 < P.nchunks; c0 += per_item) {
             const uint64_t cnt = std::min<uint64_t>(per_item, P.nchunks - c0);
-            costs.push_back({((cnt + 63) / 64) * P.L * (uint64_t)(P.NS + 4), (unsigned)items.size()});
+            costs.push_back({((cnt + walkers - 1) / walkers) * P.L * (uint64_t)(P.NS + 4), (unsigned)items.size()});
             items.push_back(RepItem{(uint32_t)k, (uint32_t)cnt, (uint32_t)c0, 0u});
         }
         task_items[k].second = (uint32_t)items.size() - task_items[k].first;
@@ -446,6 +454,8 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
         t.NS = P.NS;
         t.L = (int)P.L;
         t.nhigh = P.nhigh;
+        t.nlow = P.nlow;
+        for (int ld = 0; ld < P.nlow; ++ld) t.lrad[ld] = (uint32_t)P.low_radix[(size_t)ld];
         t.item0 = task_items[k].first;
         t.nitems = task_items[k].second;
         t.out_index = (uint32_t)k;
@@ -528,8 +538,9 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     if (ntasks < 0) return fail(PQB_ERR_ARG, "negative task count");
     if (ntasks > 0 && (!tasks || !out)) return fail(PQB_ERR_ARG, "NULL pointer");
     if (algo != PQB_ALGO_BBFG && algo != PQB_ALGO_GLYNN) return fail(PQB_ERR_ARG, "unknown algorithm code");
-    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_EXACT) return fail(PQB_ERR_ARG, "unknown precision code");
+    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_INF) return fail(PQB_ERR_ARG, "unknown precision code");
     if (precision == PQB_PREC_EXACT) return fail(PQB_ERR_ARG, "exact mode is available for the plain permanent only");
+    const bool dd = precision == PQB_PREC_INF;
     std::vector<TaskPlan> plans((size_t)ntasks);
     std::vector<long double> steps((size_t)ntasks, 0.0L);
     long double total = 0.0L;
@@ -565,12 +576,14 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     std::lock_guard<std::mutex> lk(g_eng.mu);
     int st = g_eng.ensure_init_locked(-1);
     if (st) return st;
-    const int64_t lanes = (int64_t)batch_grid_locked(2) * 256;
+    // chunks in flight: a lane each (FP64), a quad each (double-double)
+    const int64_t lanes = dd ? (int64_t)batch_grid_locked(kDDRange0 + 1) * 256 / 4 : (int64_t)batch_grid_locked(2) * 256;
     for (int64_t i = 0; i < ntasks; ++i) {
         if (plans[(size_t)i].trivial) continue;
         int64_t photons = 0;
         for (int64_t j = 0; j < tasks[i].cols; ++j) photons += tasks[i].col_mult[j];
-        const int64_t tuned = (photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons] : 0;
+        // (the autotuned walk lengths are measured for the FP64 sweep)
+        const int64_t tuned = (!dd && photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons] : 0;
         const uint64_t L = L_override > 0 ? (uint64_t)L_override
                            : tuned > 0    ? (uint64_t)tuned
                                           : choose_L(steps[(size_t)i], total, lanes);
@@ -579,12 +592,27 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
         if (st) return st;
     }
     std::vector<double> out6;
-    st = run_tasks_locked(plans, out6);
+    st = run_tasks_locked(plans, out6, dd);
     if (st) return st;
     for (int64_t i = 0; i < ntasks; ++i) {
         out[2 * i] = out6[(size_t)(6 * i)];
         out[2 * i + 1] = out6[(size_t)(6 * i + 1)];
     }
+    return PQB_OK;
+}
+
+int run_plain_dd_locked(const double* A, int R, int C, int64_t ld, double* out) {
+    const std::vector<int64_t> rm((size_t)R, 1), cm((size_t)C, 1);
+    const long double S = std::ldexp(1.0L, R - 1);
+    const int64_t quads = (int64_t)batch_grid_locked(kDDRange0 + 1) * 256 / 4;
+    std::vector<TaskPlan> plans(1);
+    int st = plan_task(A, R, C, ld, rm.data(), cm.data(), choose_L(S, S, quads), plans[0], true);
+    if (st) return st;
+    std::vector<double> out6;
+    st = run_tasks_locked(plans, out6, true);
+    if (st) return st;
+    out[0] = out6[0];
+    out[1] = out6[1];
     return PQB_OK;
 }
 

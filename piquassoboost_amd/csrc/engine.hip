@@ -341,7 +341,7 @@ int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int alg
     if (!out) return fail(PQB_ERR_ARG, "output pointer is NULL");
     if (rows < 0 || cols < 0) return fail(PQB_ERR_ARG, "negative matrix dimension");
     if (algo < PQB_ALGO_BBFG || algo > PQB_ALGO_SIMPLE) return fail(PQB_ERR_ARG, "unknown algorithm code");
-    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_EXACT)
+    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_INF)
         return fail(PQB_ERR_ARG, "unknown precision code");
     if (plain_edge(algo, rows, cols, out)) {
         if (words) {
@@ -380,6 +380,7 @@ int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int alg
         }
         return PQB_OK;
     }
+    if (precision == PQB_PREC_INF) return run_plain_dd_locked(A, R, C, ld, out);
     return run_plain_locked(A, R, C, ld, out);
 }
 

@@ -414,30 +414,9 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
     for (int e = tid; e < n * n; e += blockDim.x) out[e] = acc[e];
 }
 
-// perm(A) = sum_j a_0j G_0j (Laplace expansion along row 0 of the finished
-// gradient), products split exactly (FMA) and summed in double-double.  One
-// thread per matrix.
-__global__ void grad_perm_kernel(const double2* __restrict__ A, const double2* __restrict__ G,
-                                 double2* __restrict__ perm, int n, int count) {
-    const int task = blockIdx.x * blockDim.x + threadIdx.x;
-    if (task >= count) return;
-    const double2* a = A + (size_t)task * n * n;  // row 0
-    const double2* g = G + (size_t)task * n * n;  // row 0
-    double hr = 0.0, lr = 0.0, hi = 0.0, li = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const double2 x = a[j], y = g[j];
-        // re = x.x y.x - x.y y.y, im = x.x y.y + x.y y.x, each product as p + e exactly
-        const double p1 = x.x * y.x, e1 = fma(x.x, y.x, -p1);
-        const double p2 = -x.y * y.y, e2 = fma(-x.y, y.y, -p2);
-        const double p3 = x.x * y.y, e3 = fma(x.x, y.y, -p3);
-        const double p4 = x.y * y.x, e4 = fma(x.y, y.x, -p4);
-        dd_add(hr, lr, p1, e1);
-        dd_add(hr, lr, p2, e2);
-        dd_add(hi, li, p3, e3);
-        dd_add(hi, li, p4, e4);
-    }
-    perm[task] = make_double2(hr + lr, hi + li);
-}
+// (perm(A) = sum_j a_0j G_0j, the Laplace expansion along row 0 of the
+// finished gradient, is grad_perm_kernel in perm_dd.hip: its exact product
+// splits need a translation unit without contraction.)
 
 // Sum of the per-block partials of each task (double-double, fixed order),
 // scaled.  Block = 64 elements x 4 slices of the partial index.
@@ -502,12 +481,6 @@ hipError_t launch_grad(int M, int K, bool minors, const GradParams& p, int nbloc
 hipError_t launch_grad_reduce(const GradReduceParams& p, hipStream_t st) {
     const int64_t chunks = (p.main + (p.tail ? 1 : 0) + 63) / 64;
     hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)(chunks * p.count)), dim3(256), 0, st, p);
-    return hipGetLastError();
-}
-
-hipError_t launch_grad_perm(const double2* A, const double2* G, double2* perm, int n, int count, hipStream_t st) {
-    const int bd = 64;
-    hipLaunchKernelGGL(grad_perm_kernel, dim3((count + bd - 1) / bd), dim3(bd), 0, st, A, G, perm, n, count);
     return hipGetLastError();
 }
 

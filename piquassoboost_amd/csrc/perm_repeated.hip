@@ -274,6 +274,10 @@ hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks,
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    return launch_rep_reduce(p, st);
+}
+
+hipError_t launch_rep_reduce(const RepBatchParams& p, hipStream_t st) {
     hipLaunchKernelGGL(rep_reduce_kernel, dim3(kReduceBlocks), dim3(64), 0, st, p);
     return hipGetLastError();
 }

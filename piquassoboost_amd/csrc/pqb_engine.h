@@ -44,9 +44,11 @@ struct TaskPlan {
 };
 
 int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* rmult,
-              const int64_t* cmult, uint64_t target_L, TaskPlan& P);
+              const int64_t* cmult, uint64_t target_L, TaskPlan& P, bool plain = false);
 int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, double* out,
                 int64_t L_override);
+// Plain permanent (1 <= rows <= cols + 1) by the double-double sweep; engine lock held.
+int run_plain_dd_locked(const double* A, int R, int C, int64_t ld, double* out);
 
 struct Engine {
     std::mutex mu;

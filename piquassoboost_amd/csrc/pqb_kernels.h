@@ -53,7 +53,7 @@ struct RepTask {
     int NS;                // column slots
     int L;                 // low-walk length = prod_low (m_i + 1)
     int nhigh;             // high digits
-    int pad0;
+    int nlow;              // low digits (rows 1 .. nlow of A)
     uint32_t item0;        // first work item of this task (items of a task are contiguous)
     uint32_t nitems;
     uint32_t out_index;    // result slot
@@ -61,6 +61,8 @@ struct RepTask {
     double scale;          // 2^-(N-1)
     uint32_t hrow[kMaxDigits];  // row (in A) of high digit h
     uint32_t hrad[kMaxDigits];  // radix m_h + 1 of high digit h (digit 0 varies fastest)
+    uint32_t lrad[kMaxDigits];  // radix m_d + 1 of low digit d (row 1 + d); read by the
+                                // double-double sweep, which seeds chunks from A exactly
 };
 
 struct RepItem {
@@ -89,6 +91,16 @@ constexpr int kReduceBlocks = 512;  // fixed grid of the per-task reduction (one
 int rep_range_for(int min_ns, int max_ns);
 const void* rep_batch_kernel_fn(int range);
 hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks, hipStream_t st);
+// per-task sum of the item partials (shared by the FP64 and double-double sweeps)
+hipError_t launch_rep_reduce(const RepBatchParams& p, hipStream_t st);
+
+// Double-double sweep over the same tasks and items (precision PQB_PREC_INF).
+// Ranges kDDRange0 .. kDDRange0 + 2 cover up to 16, 32 and 40 slots.
+constexpr int kDDRange0 = 5;
+constexpr int kRanges = kDDRange0 + 3;
+int rep_dd_range_for(int max_ns);
+const void* rep_dd_kernel_fn(int range);
+hipError_t launch_rep_dd(int range, const RepBatchParams& p, int grid_blocks, hipStream_t st);
 
 // ---- adjoint sweep (permanent + all first derivatives of square matrices) ----
 // Adjoint mode: square n x n matrices.  Minors mode: n x (n+1) matrices, the

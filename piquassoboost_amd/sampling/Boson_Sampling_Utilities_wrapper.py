@@ -48,7 +48,7 @@ GlynnSimpleLongDouble = 11
 # lib code -> (algorithm, precision) of the engine
 PLAIN_LIBS = {
     GlynnLongDouble: (C.ALGO_GLYNN, C.PREC_EXTENDED),
-    GlynnInf: (C.ALGO_BBFG, C.PREC_EXTENDED),
+    GlynnInf: (C.ALGO_BBFG, C.PREC_INF),
     GlynnDouble: (C.ALGO_GLYNN, C.PREC_DOUBLE),
     BBFGPermanentCalculatorDouble: (C.ALGO_BBFG, C.PREC_DOUBLE),
     BBFGPermanentCalculatorLongDouble: (C.ALGO_BBFG, C.PREC_EXTENDED),
@@ -76,7 +76,7 @@ REPEATED_LIBS = {
     GlynnRepCPUDouble: (C.ALGO_GLYNN, C.PREC_DOUBLE),
     BBFGPermanentCalculatorRepeatedDouble: (C.ALGO_BBFG, C.PREC_DOUBLE),
     BBFGPermanentCalculatorRepeatedLongDouble: (C.ALGO_BBFG, C.PREC_EXTENDED),
-    GlynnRepInf: (C.ALGO_BBFG, C.PREC_EXTENDED),
+    GlynnRepInf: (C.ALGO_BBFG, C.PREC_INF),
 }
 
 

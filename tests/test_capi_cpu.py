@@ -66,10 +66,25 @@ def test_argument_errors():
     assert L.pqb_perm(A.ctypes.data_as(C._dp), 2, 2, 1, 0, 0, out) == C.PQB_ERR_ARG  # ld < cols
     assert L.pqb_perm(A.ctypes.data_as(C._dp), 2, 2, 2, 9, 0, out) == C.PQB_ERR_ARG  # algo
     assert L.pqb_perm(A.ctypes.data_as(C._dp), 2, 2, 2, 0, 9, out) == C.PQB_ERR_ARG  # precision
+    assert L.pqb_perm(A.ctypes.data_as(C._dp), 2, 2, 2, 0, C.PREC_INF + 1, out) == C.PQB_ERR_ARG
     assert L.pqb_perm(None, 2, 2, 2, 0, 0, out) == C.PQB_ERR_ARG
     big = np.ones((41, 41), dtype=np.complex128)
     assert L.pqb_perm(big.ctypes.data_as(C._dp), 41, 41, 41, 0, 0, out) == C.PQB_ERR_SIZE
     assert "40 columns" in L.pqb_last_error().decode()
+
+
+def test_inf_precision_edge_rules_need_no_device():
+    """PQB_PREC_INF follows the same shape rules; decided cases never touch the device."""
+    assert _raw_perm(np.zeros((0, 0)), C.ALGO_BBFG, C.PREC_INF) == (0, 1.0)
+    assert _raw_perm(np.ones((5, 3)), C.ALGO_BBFG, C.PREC_INF) == (0, 0.0)
+    z = np.zeros(3, dtype=np.int64)
+    assert C.permanent_batch([(np.eye(3), z, z)], C.ALGO_BBFG, C.PREC_INF) == [1.0 + 0.0j]
+    with pytest.raises(Exception, match="Number of input and output states should be equal"):
+        C.permanent_repeated(np.eye(2), np.array([1, 1]), np.array([2, 1]), C.ALGO_BBFG, C.PREC_INF)
+    from piquassoboost_amd.sampling import Boson_Sampling_Utilities_wrapper as W
+
+    assert W.PLAIN_LIBS[W.GlynnInf] == (C.ALGO_BBFG, C.PREC_INF)
+    assert W.REPEATED_LIBS[W.GlynnRepInf] == (C.ALGO_BBFG, C.PREC_INF)
 
 
 def test_exact_mode_rejects_non_integer():

@@ -40,7 +40,8 @@ def test_rectangular_vs_oracle(shape):
     A = random_complex(rng, r, c)
     ref = oracle.bbfg(A, "dd")
     got = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
-    # rows = cols + 1 sums to zero up to rounding: compare on the scale of the terms
+    # rows = cols + 1: the sum is perm(A without row 0) and its terms cancel more
+    # than in the square case; compare on the scale of the terms
     scale = max(abs(ref), np.prod(np.abs(A).sum(axis=0)) * 1e-3)
     assert abs(got - ref) <= TOL_SMALL * scale, (got, ref)
 
