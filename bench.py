@@ -246,6 +246,39 @@ def sampling_leg(photons: int = 20, modes: int = 60, shots: int = 3) -> dict:
     }
 
 
+def inf_leg(fp64_ms_n30: float, reps: int = 5) -> dict:
+    """Double-double sweep (PQB_PREC_INF, the exact-arithmetic calculator codes):
+    n=20 (40-mode Haar, seed 1001) and n=30 (the headline block) per permanent.
+    Device time from replays of the captured launch (sweep + reduction); the
+    host figure is the oracle's double-double restatement of the reference
+    sweep on a bounded sample of the n=20 permanent."""
+    from piquassoboost_amd import _capi as C
+    from piquassoboost_amd.sampling.utilities import haar_unitary
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+
+    out = {}
+    for n, m, seed in ((20, 40, 1001), (30, M_MODES, SEED)):
+        A = np.ascontiguousarray(haar_unitary(m, seed)[:n, :n])
+        v = C.permanent(A, C.ALGO_BBFG, C.PREC_INF)
+        ms = C.batch_replay_ms(reps) / reps
+        out[f"n{n}_ms"] = ms
+        out[f"n{n}_perm"] = [v.real, v.imag]
+    out["n30_vs_fp64_sweep"] = out["n30_ms"] / fp64_ms_n30
+    A = np.ascontiguousarray(haar_unitary(40, 1001)[:20, :20])
+    threads = max(1, min(16, os.cpu_count() or 1))
+    njobs = 64 * threads
+    jobs = 4 * threads
+    t0 = time.perf_counter()
+    oracle.bbfg_partial(A, njobs, 0, jobs, "dd", threads)
+    dt = time.perf_counter() - t0
+    out["host_dd_n20_ms"] = dt * njobs / jobs * 1e3
+    out["host_threads"] = threads
+    out["gpu_vs_host_dd_n20"] = out["host_dd_n20_ms"] / out["n20_ms"]
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +288,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the forward+backward leg")
     ap.add_argument("--no-sampling", action="store_true", help="skip the boson-sampling leg")
+    ap.add_argument("--no-inf", action="store_true", help="skip the double-double (PQB_PREC_INF) leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
                          "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
@@ -353,6 +387,8 @@ def main() -> int:
         line["grad"] = grad_leg()
     if not a.no_sampling:
         line["sampling"] = sampling_leg()
+    if not a.no_inf:
+        line["inf"] = inf_leg(ms_per_step)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A)
     print(json.dumps(line), flush=True)

@@ -398,10 +398,23 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     std::vector<std::pair<uint32_t, uint32_t>> task_items(live.size());
     const uint64_t lane_steps = 2048;     // target Gray steps per lane (quad) per item
     const uint64_t walkers = dd ? 16 : 64;  // chunks a wave works on at once
+    auto per_item_of = [&](const TaskPlan& P) { return walkers * std::max<uint64_t>(1, lane_steps / P.L); };
+    // A batch too small to give every resident wave ~4 items at that size
+    // (a single mid-size task, say) gets smaller items, down to one chunk per
+    // walker, so that the whole grid has work.
+    uint64_t cap = ~(uint64_t)0;
+    {
+        const uint64_t target = 4 * (uint64_t)batch_grid_locked(range) * (kBlock / kWave);
+        uint64_t n_default = 0, chunks = 0;
+        for (size_t k : live) {
+            n_default += (plans[k].nchunks + per_item_of(plans[k]) - 1) / per_item_of(plans[k]);
+            chunks += plans[k].nchunks;
+        }
+        if (n_default < target) cap = walkers * std::max<uint64_t>(1, (chunks + target * walkers - 1) / (target * walkers));
+    }
     for (size_t k = 0; k < live.size(); ++k) {
         const TaskPlan& P = plans[live[k]];
-        const uint64_t per_lane = std::max<uint64_t>(1, lane_steps / P.L);
-        const uint64_t per_item = walkers * per_lane;
+        const uint64_t per_item = std::min(per_item_of(P), cap);
         task_items[k].first = (uint32_t)items.size();
         for (uint64_t c0 = 0; c0 < P.nchunks; c0 += per_item) {
             const uint64_t cnt = std::min<uint64_t>(per_item, P.nchunks - c0);
