@@ -84,7 +84,7 @@ def batched_leg() -> dict:
         s = p["walk"] * p["chunks"]
         steps += s
         flops += s * (8 * p["slots"] - 2)
-    C.permanent_batch(tasks[:64])  # warm: buffers, graph capture
+    C.permanent_batch(tasks)  # warm: buffers sized for the batch, graph capture
     C.permanent_repeated(*tasks[0])
     t0 = time.perf_counter()
     C.permanent_batch(tasks)

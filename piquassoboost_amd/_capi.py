@@ -189,18 +189,14 @@ def permanent_repeated(A, row_mult, col_mult, algo: int = ALGO_BBFG, precision: 
     return complex(out[0], out[1])
 
 
-class _Task(ctypes.Structure):
-    """struct pqb_task of include/pqb_perm.h"""
-
-    _fields_ = [("A", _dp), ("rows", ctypes.c_int64), ("cols", ctypes.c_int64), ("ld", ctypes.c_int64),
-                ("row_mult", _i64p), ("col_mult", _i64p)]
-
-
 def _task_array(tasks):
-    """tasks: iterable of (matrix, row_mult, col_mult).  Returns the ctypes array
-    and the numpy arrays it points into (kept alive by the caller)."""
+    """tasks: sequence of (matrix, row_mult, col_mult).  Returns an (n, 6) int64
+    array laid out as n structs pqb_task of include/pqb_perm.h (six 8-byte
+    fields: A, rows, cols, ld, row_mult, col_mult; pointers as addresses) and the
+    arrays it points into (kept alive by the caller).  Inputs that are already
+    C-contiguous complex128 / int64 are used in place."""
     keep = []
-    arr = (_Task * len(tasks))()
+    recs = []
     for i, (A, r, c) in enumerate(tasks):
         M = as_matrix(A)
         rv = as_state(r)
@@ -209,10 +205,10 @@ def _task_array(tasks):
             raise Exception(
                 f"task {i}: state lengths must match the matrix: len(output_state) == rows, len(input_state) == cols"
             )
-        keep += [M, rv, cv]
-        arr[i] = _Task(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1], rv.ctypes.data_as(_i64p),
-                       cv.ctypes.data_as(_i64p))
-    return arr, keep
+        keep.append((M, rv, cv))
+        recs.append((M.__array_interface__["data"][0], M.shape[0], M.shape[1], M.shape[1],
+                     rv.__array_interface__["data"][0], cv.__array_interface__["data"][0]))
+    return np.array(recs, dtype=np.int64).reshape(-1, 6), keep
 
 
 def permanent_batch(tasks, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE, walk_length: int = 0) -> list:
@@ -221,14 +217,14 @@ def permanent_batch(tasks, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE, 
     if not tasks:
         return []
     arr, keep = _task_array(tasks)
-    out = (ctypes.c_double * (2 * len(tasks)))()
+    out = np.zeros(2 * len(tasks), dtype=np.float64)
+    tp = ctypes.c_void_p(arr.ctypes.data)
     if walk_length:
-        _check(lib().pqb_perm_batch_tuned(ctypes.cast(arr, ctypes.c_void_p), len(tasks), algo, precision,
-                                          walk_length, out))
+        _check(lib().pqb_perm_batch_tuned(tp, len(tasks), algo, precision, walk_length, out.ctypes.data_as(_dp)))
     else:
-        _check(lib().pqb_perm_batch(ctypes.cast(arr, ctypes.c_void_p), len(tasks), algo, precision, out))
+        _check(lib().pqb_perm_batch(tp, len(tasks), algo, precision, out.ctypes.data_as(_dp)))
     del keep
-    return [complex(out[2 * i], out[2 * i + 1]) for i in range(len(tasks))]
+    return (out[0::2] + 1j * out[1::2]).tolist()
 
 
 def set_walk_lengths(table) -> None:

@@ -29,6 +29,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pqb_perm.h"
@@ -591,19 +592,43 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     if (st) return st;
     // chunks in flight: a lane each (FP64), a quad each (double-double)
     const int64_t lanes = dd ? (int64_t)batch_grid_locked(kDDRange0 + 1) * 256 / 4 : (int64_t)batch_grid_locked(2) * 256;
-    for (int64_t i = 0; i < ntasks; ++i) {
-        if (plans[(size_t)i].trivial) continue;
-        int64_t photons = 0;
-        for (int64_t j = 0; j < tasks[i].cols; ++j) photons += tasks[i].col_mult[j];
-        // (the autotuned walk lengths are measured for the FP64 sweep)
-        const int64_t tuned = (!dd && photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons] : 0;
-        const uint64_t L = L_override > 0 ? (uint64_t)L_override
-                           : tuned > 0    ? (uint64_t)tuned
-                                          : choose_L(steps[(size_t)i], total, lanes);
-        st = plan_task(tasks[i].A, tasks[i].rows, tasks[i].cols, tasks[i].ld, tasks[i].row_mult, tasks[i].col_mult,
-                       L, plans[(size_t)i]);
-        if (st) return st;
+    // Plans are independent: large batches are planned on several host
+    // threads (a plan is a few microseconds; 4096 of them dominate the host
+    // side of a launch otherwise).  Errors come back per task; the first one
+    // by task index is reported, as the serial loop would.
+    std::vector<int> pst((size_t)ntasks, PQB_OK);
+    std::vector<std::string> perr((size_t)ntasks);
+    auto plan_range = [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            if (plans[(size_t)i].trivial) continue;
+            int64_t photons = 0;
+            for (int64_t j = 0; j < tasks[i].cols; ++j) photons += tasks[i].col_mult[j];
+            // (the autotuned walk lengths are measured for the FP64 sweep)
+            const int64_t tuned = (!dd && photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons] : 0;
+            const uint64_t L = L_override > 0 ? (uint64_t)L_override
+                               : tuned > 0    ? (uint64_t)tuned
+                                              : choose_L(steps[(size_t)i], total, lanes);
+            pst[(size_t)i] = plan_task(tasks[i].A, tasks[i].rows, tasks[i].cols, tasks[i].ld, tasks[i].row_mult,
+                                       tasks[i].col_mult, L, plans[(size_t)i]);
+            if (pst[(size_t)i]) perr[(size_t)i] = t_err;  // t_err is per thread
+        }
+    };
+    const int64_t per_thread = 256;
+    const int64_t nthreads = std::min<int64_t>(
+        std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency()), std::min<int64_t>(16, (ntasks + per_thread - 1) / per_thread));
+    if (nthreads <= 1) {
+        plan_range(0, ntasks);
+    } else {
+        std::vector<std::thread> pool;
+        for (int64_t t = 0; t < nthreads; ++t)
+            pool.emplace_back(plan_range, ntasks * t / nthreads, ntasks * (t + 1) / nthreads);
+        for (std::thread& th : pool) th.join();
     }
+    for (int64_t i = 0; i < ntasks; ++i)
+        if (pst[(size_t)i]) {
+            t_err = (ntasks > 1 ? "task " + std::to_string(i) + ": " : std::string()) + perr[(size_t)i];
+            return pst[(size_t)i];
+        }
     std::vector<double> out6;
     st = run_tasks_locked(plans, out6, dd);
     if (st) return st;

@@ -96,6 +96,27 @@ def test_c2a_haar_large(haar_blocks, name):
     assert err < 1e-10
 
 
+def test_largest_size_n40_block_diagonal():
+    """The largest order the kernels are built for (40 columns, 2^39 Gray steps): a
+    block-diagonal of two n=20 Haar blocks with rows and columns permuted has
+    perm = perm(B1) perm(B2) (the factors from the double-double oracle).  A lane
+    sums ~2^24 FP64 terms here; the bar is 1e-8 (1e-10 is stated up to n=30)."""
+    from piquassoboost_amd.sampling.utilities import haar_unitary
+
+    B1 = haar_unitary(40, 2001)[:20, :20]
+    B2 = haar_unitary(40, 2002)[:20, :20]
+    A = np.zeros((40, 40), dtype=np.complex128)
+    A[:20, :20] = B1
+    A[20:, 20:] = B2
+    rng = np.random.default_rng(2003)
+    A = A[rng.permutation(40)][:, rng.permutation(40)]
+    ref = oracle.bbfg(B1, "dd") * oracle.bbfg(B2, "dd")
+    got = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
+    err = rel_err(got, ref)
+    print(f"n=40: got={got!r} ref={ref!r} rel_err={err:.3e}", flush=True)
+    assert err < 1e-8
+
+
 def test_exact_small_integer_matrices():
     rng = np.random.default_rng(21)
     for n in range(1, 15):
