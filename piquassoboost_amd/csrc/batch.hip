@@ -29,6 +29,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -608,9 +609,14 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
             const uint64_t L = L_override > 0 ? (uint64_t)L_override
                                : tuned > 0    ? (uint64_t)tuned
                                               : choose_L(steps[(size_t)i], total, lanes);
-            pst[(size_t)i] = plan_task(tasks[i].A, tasks[i].rows, tasks[i].cols, tasks[i].ld, tasks[i].row_mult,
-                                       tasks[i].col_mult, L, plans[(size_t)i]);
-            if (pst[(size_t)i]) perr[(size_t)i] = t_err;  // t_err is per thread
+            try {  // nothing may escape a worker thread
+                pst[(size_t)i] = plan_task(tasks[i].A, tasks[i].rows, tasks[i].cols, tasks[i].ld,
+                                           tasks[i].row_mult, tasks[i].col_mult, L, plans[(size_t)i]);
+                if (pst[(size_t)i]) perr[(size_t)i] = t_err;  // t_err is per thread
+            } catch (const std::exception& ex) {
+                pst[(size_t)i] = PQB_ERR_DEVICE;
+                perr[(size_t)i] = std::string("internal error: ") + ex.what();
+            }
         }
     };
     const int64_t per_thread = 256;
@@ -620,8 +626,15 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
         plan_range(0, ntasks);
     } else {
         std::vector<std::thread> pool;
-        for (int64_t t = 0; t < nthreads; ++t)
-            pool.emplace_back(plan_range, ntasks * t / nthreads, ntasks * (t + 1) / nthreads);
+        pool.reserve((size_t)nthreads);
+        for (int64_t t = 0; t < nthreads; ++t) {
+            const int64_t lo = ntasks * t / nthreads, hi = ntasks * (t + 1) / nthreads;
+            try {
+                pool.emplace_back(plan_range, lo, hi);
+            } catch (const std::system_error&) {
+                plan_range(lo, hi);  // no thread to be had: plan this slice here
+            }
+        }
         for (std::thread& th : pool) th.join();
     }
     for (int64_t i = 0; i < ntasks; ++i)
