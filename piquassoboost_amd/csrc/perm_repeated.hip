@@ -92,15 +92,19 @@ __device__ __forceinline__ void add_wterm(const double (&cr)[NS], const double (
     }
 }
 
-// colsum += S * row with S = +/-2 an inline constant (the direction is
-// wave-uniform, so the caller branches uniformly between the two forms).
-template <int NS, int S>
-__device__ __forceinline__ void rep_add_row(double (&cr)[NS], double (&ci)[NS], cptr<double2> row) {
+// colsum += f * row, f = +/-2.  The row is an SGPR operand (scalar loads); a
+// VALU instruction reads at most one SGPR here, so the wave-uniform factor is
+// kept in a VGPR (one move per step) rather than branching between two
+// inline-constant forms: the two-path loop needed 224 VGPRs at 26 slots
+// against 149 for this one (3 waves per SIMD instead of 2).
+template <int NS>
+__device__ __forceinline__ void rep_add_row(double (&cr)[NS], double (&ci)[NS], cptr<double2> row, double f) {
+    asm volatile("" : "+v"(f));
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
         const double2 v = row[j];
-        cr[j] = fma((double)S, v.x, cr[j]);
-        ci[j] = fma((double)S, v.y, ci[j]);
+        cr[j] = fma(f, v.x, cr[j]);
+        ci[j] = fma(f, v.y, ci[j]);
         if ((j & 15) == 15) __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -153,10 +157,7 @@ __device__ __forceinline__ void rep_chunk(cptr<RepTask> T, const double* __restr
         for (int t = 1; t < L; ++t) {
             const RepStep en = tab[t + 1 < L ? t + 1 : t];  // next entry, loaded ahead
             const cptr<double2> row = A + (size_t)e.row * NS;
-            if (e.dir)
-                rep_add_row<NS, -2>(cr, ci, row);
-            else
-                rep_add_row<NS, 2>(cr, ci, row);
+            rep_add_row<NS>(cr, ci, row, e.dir ? -2.0 : 2.0);
             add_wterm<NS>(cr, ci, e.w, ar, ai);
             e = en;
         }
@@ -186,9 +187,11 @@ __device__ __forceinline__ void rep_dispatch(int ns, cptr<RepTask> T, const doub
 
 // Persistent grid over the work items; slot counts in [LO, HI].  Registers
 // are allocated for the largest case, so the host picks the narrowest range
-// that covers the batch.
+// that covers the batch.  Up to 32 slots the kernel fits 168 VGPRs: three
+// waves per SIMD.
 template <int LO, int HI>
-__global__ __launch_bounds__(256) void rep_batch_kernel(RepBatchParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HI <= 32 ? 3 : 1)))
+void rep_batch_kernel(RepBatchParams p) {
     const int lane = threadIdx.x & (kWave - 1);
     const cptr<RepItem> items = to_const(p.items);
     const cptr<unsigned> order = to_const(p.order);
