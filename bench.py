@@ -259,7 +259,7 @@ def inf_leg(fp64_ms_n30: float, reps: int = 5) -> dict:
     import oracle
 
     out = {}
-    for n, m, seed in ((20, 40, 1001), (30, M_MODES, SEED)):
+    for n, m, seed in ((16, 32, 1001), (20, 40, 1001), (30, M_MODES, SEED)):
         A = np.ascontiguousarray(haar_unitary(m, seed)[:n, :n])
         v = C.permanent(A, C.ALGO_BBFG, C.PREC_INF)
         ms = C.batch_replay_ms(reps) / reps

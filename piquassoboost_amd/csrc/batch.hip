@@ -399,7 +399,7 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     std::vector<ItemCost> costs;
     std::vector<std::pair<uint32_t, uint32_t>> task_items(live.size());
     const uint64_t lane_steps = 2048;     // target Gray steps per lane (quad) per item
-    const uint64_t walkers = dd ? 16 : 64;  // chunks a wave works on at once
+    const uint64_t walkers = dd ? (uint64_t)rep_dd_walkers(range) : 64;  // chunks a wave works on at once
     auto per_item_of = [&](const TaskPlan& P) { return walkers * std::max<uint64_t>(1, lane_steps / P.L); };
     // A batch too small to give every resident wave ~4 items at that size
     // (a single mid-size task, say) gets smaller items, down to one chunk per
@@ -559,6 +559,7 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     std::vector<TaskPlan> plans((size_t)ntasks);
     std::vector<long double> steps((size_t)ntasks, 0.0L);
     long double total = 0.0L;
+    int max_photons = 1;  // = the largest slot count of the batch
     for (int64_t i = 0; i < ntasks; ++i) {
         bool decided = false;
         double v = 1.0;
@@ -578,6 +579,7 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
         cm.erase(std::remove(cm.begin(), cm.end(), 0), cm.end());
         steps[(size_t)i] = std::min(sweep_steps(rm), sweep_steps(cm));
         total += steps[(size_t)i];
+        max_photons = std::max(max_photons, (int)std::accumulate(rm.begin(), rm.end(), (int64_t)0));
     }
     bool any_sweep = false;
     for (const TaskPlan& P : plans) any_sweep = any_sweep || !P.trivial;
@@ -591,8 +593,10 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     std::lock_guard<std::mutex> lk(g_eng.mu);
     int st = g_eng.ensure_init_locked(-1);
     if (st) return st;
-    // chunks in flight: a lane each (FP64), a quad each (double-double)
-    const int64_t lanes = dd ? (int64_t)batch_grid_locked(kDDRange0 + 1) * 256 / 4 : (int64_t)batch_grid_locked(2) * 256;
+    // chunks in flight: a lane each (FP64), a group of lanes each (double-double)
+    const int dd_range = rep_dd_range_for(max_photons);
+    const int64_t lanes = dd ? (int64_t)batch_grid_locked(dd_range) * (kBlock / kWave) * rep_dd_walkers(dd_range)
+                             : (int64_t)batch_grid_locked(2) * kBlock;
     // Plans are independent: large batches are planned on several host
     // threads (a plan is a few microseconds; 4096 of them dominate the host
     // side of a launch otherwise).  Errors come back per task; the first one
@@ -655,9 +659,10 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
 int run_plain_dd_locked(const double* A, int R, int C, int64_t ld, double* out) {
     const std::vector<int64_t> rm((size_t)R, 1), cm((size_t)C, 1);
     const long double S = std::ldexp(1.0L, R - 1);
-    const int64_t quads = (int64_t)batch_grid_locked(kDDRange0 + 1) * 256 / 4;
+    const int range = rep_dd_range_for(C);  // the slots are the columns
+    const int64_t walkers = (int64_t)batch_grid_locked(range) * (kBlock / kWave) * rep_dd_walkers(range);
     std::vector<TaskPlan> plans(1);
-    int st = plan_task(A, R, C, ld, rm.data(), cm.data(), choose_L(S, S, quads), plans[0], true);
+    int st = plan_task(A, R, C, ld, rm.data(), cm.data(), choose_L(S, S, walkers), plans[0], true);
     if (st) return st;
     std::vector<double> out6;
     st = run_tasks_locked(plans, out6, true);

@@ -23,10 +23,16 @@
 // unless the sum cancels by more than ~2^45.
 //
 // Layout: the tasks, step tables, work items and persistent grid of the FP64
-// batched sweep (perm_repeated.hip).  A chunk is handled by a quad of lanes:
-// lane q holds the column sums of slots q*M .. q*M+M-1 (M = ceil(NS/4), four
-// doubles per slot), multiplies them, and the quad combines its four partial
-// products with two DPP quad permutations.  A wave works on 16 chunks at a time.
+// batched sweep (perm_repeated.hip).  A chunk is handled by a group of G = 1,
+// 2 or 4 lanes: lane q of the group holds the column sums of slots
+// q*M .. q*M+M-1 (M = ceil(NS/G), four doubles per slot), multiplies them in
+// two independent chains, and the group combines its partial products with
+// DPP quad permutations (one for a pair, two for a quad).  A wave works on
+// 64/G chunks at a time.  G grows with the slot count so that a lane holds at
+// most 8 slots, which fits the 168 VGPRs of three waves per SIMD: G = 1 up to
+// 8 slots, 2 up to 16, 4 up to 32 (each exchange repeats a product on every
+// lane of the group, so smaller groups do less work); 33..40 slots take 10 per
+// lane at two waves per SIMD.
 //
 // The file also holds grad_perm_kernel (the permanent from the finished
 // gradient by a double-double Laplace sum).
@@ -136,26 +142,41 @@ __device__ __forceinline__ void ddc_acc_w(ddc& acc, double w, const ddc& x) {
     }
 }
 
-// Product of all column sums of the chunk: the lane's M slots, then the quad.
-template <int M>
-__device__ __forceinline__ ddc quad_product(const double (&hr)[M], const double (&lr)[M], const double (&hi)[M],
-                                            const double (&li)[M]) {
+// Product of all column sums of the chunk: the lane's M slots as two
+// independent chains (same number of products, twice the ILP), then across
+// the group of G lanes.
+template <int G, int M>
+__device__ __forceinline__ ddc group_product(const double (&hr)[M], const double (&lr)[M], const double (&hi)[M],
+                                             const double (&li)[M]) {
+    constexpr int H = M / 2;  // second chain starts at slot H (M >= 2)
     ddc p = {hr[0], lr[0], hi[0], li[0]};
+    if constexpr (M >= 2) {
+        ddc u = {hr[H], lr[H], hi[H], li[H]};
 #pragma unroll
-    for (int k = 1; k < M; ++k) {
-        const ddc c = {hr[k], lr[k], hi[k], li[k]};
-        ddc_mul(p, c);
+        for (int k = 1; k < M; ++k) {
+            if (k == H) continue;
+            const ddc c = {hr[k], lr[k], hi[k], li[k]};
+            if (k < H)
+                ddc_mul(p, c);
+            else
+                ddc_mul(u, c);
+        }
+        ddc_mul(p, u);
     }
-    ddc x = quad_move<kXor1>(p);
-    ddc_mul(p, x);  // product of the lane pair
-    x = quad_move<kXor2>(p);
-    ddc_mul(p, x);  // product of the quad
+    if constexpr (G >= 2) {
+        const ddc x = quad_move<kXor1>(p);
+        ddc_mul(p, x);  // product of the lane pair
+    }
+    if constexpr (G == 4) {
+        const ddc x = quad_move<kXor2>(p);
+        ddc_mul(p, x);  // product of the quad
+    }
     return p;
 }
 
-// Weighted double-double sum over one chunk (every lane of the quad ends with
-// the same value; lane q = 0's copy is used).
-template <int M>
+// Weighted double-double sum over one chunk (every lane of the group ends
+// with the same value; lane q = 0's copy is used).
+template <int G, int M>
 __device__ __forceinline__ void dd_chunk(cptr<RepTask> T, const double* __restrict__ binom, uint32_t chunk, int q,
                                          ddc& acc) {
     const int NS = T->NS;
@@ -206,7 +227,7 @@ __device__ __forceinline__ void dd_chunk(cptr<RepTask> T, const double* __restri
         const double b = binom[m * kBinomStride + kd];
         wc *= (kd & 1u) ? -b : b;
     }
-    ddc_acc_w(acc, wc, quad_product<M>(hr, lr, hi, li));
+    ddc_acc_w(acc, wc, group_product<G, M>(hr, lr, hi, li));
     const cptr<RepStep> tab = to_const(T->table);
     const int L = T->L;
     for (int t = 1; t < L; ++t) {
@@ -219,29 +240,31 @@ __device__ __forceinline__ void dd_chunk(cptr<RepTask> T, const double* __restri
             dd_add_d(hr[k], lr[k], f * a.x);
             dd_add_d(hi[k], li[k], f * a.y);
         }
-        ddc_acc_w(acc, wc * e.w, quad_product<M>(hr, lr, hi, li));
+        ddc_acc_w(acc, wc * e.w, group_product<G, M>(hr, lr, hi, li));
     }
 }
 
-template <int MLO, int MHI>
+template <int G, int MLO, int MHI>
 __device__ __forceinline__ void dd_dispatch(int m, cptr<RepTask> T, const double* __restrict__ binom, uint32_t chunk0,
-                                            uint32_t count, int q, int quad, ddc& acc) {
+                                            uint32_t count, int q, int walker, ddc& acc) {
     if constexpr (MLO <= MHI) {
         if (m == MLO) {
-            for (uint32_t j = (uint32_t)quad; j < count; j += kWave / 4) dd_chunk<MLO>(T, binom, chunk0 + j, q, acc);
+            for (uint32_t j = (uint32_t)walker; j < count; j += kWave / G)
+                dd_chunk<G, MLO>(T, binom, chunk0 + j, q, acc);
         } else {
-            dd_dispatch<MLO + 1, MHI>(m, T, binom, chunk0, count, q, quad, acc);
+            dd_dispatch<G, MLO + 1, MHI>(m, T, binom, chunk0, count, q, walker, acc);
         }
     }
 }
 
 // Persistent grid over the work items (same item protocol as rep_batch_kernel);
-// slot counts up to 4 * MHI.
-template <int MHI>
-__global__ __launch_bounds__(256) void rep_dd_kernel(RepBatchParams p) {
+// a chunk per group of G lanes, slot counts up to G * MHI.
+template <int G, int MHI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHI <= 8 ? 3 : 1))) void rep_dd_kernel(
+    RepBatchParams p) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int q = lane & 3;
-    const int quad = lane >> 2;
+    const int q = lane & (G - 1);
+    const int walker = lane / G;
     const cptr<RepItem> items = to_const(p.items);
     const cptr<unsigned> order = to_const(p.order);
     const cptr<RepTask> tasks = to_const(p.tasks);
@@ -255,7 +278,7 @@ __global__ __launch_bounds__(256) void rep_dd_kernel(RepBatchParams p) {
         const RepItem it = items[idx];
         const cptr<RepTask> T = tasks + it.task;
         ddc acc = {0.0, 0.0, 0.0, 0.0};
-        dd_dispatch<1, MHI>((T->NS + 3) / 4, T, p.binom, it.chunk0, it.count, q, quad, acc);
+        dd_dispatch<G, 1, MHI>((T->NS + G - 1) / G, T, p.binom, it.chunk0, it.count, q, walker, acc);
         if (q != 0) acc = ddc{0.0, 0.0, 0.0, 0.0};
         const ddc v = wave_reduce(acc);
         if (lane == 0) p.partials[idx] = v;
@@ -295,26 +318,38 @@ hipError_t launch_grad_perm(const double2* A, const double2* G, double2* perm, i
     return hipGetLastError();
 }
 
+// Ranges: lanes per chunk G and slots per lane, by the batch's largest slot count.
 int rep_dd_range_for(int max_ns) {
-    if (max_ns <= 16) return kDDRange0;
-    if (max_ns <= 32) return kDDRange0 + 1;
-    return kDDRange0 + 2;
+    if (max_ns <= 8) return kDDRange0;       // G = 1, up to 8 slots
+    if (max_ns <= 16) return kDDRange0 + 1;  // G = 2, up to 8 per lane
+    if (max_ns <= 32) return kDDRange0 + 2;  // G = 4, up to 8 per lane
+    return kDDRange0 + 3;                    // G = 4, up to 10 per lane
+}
+
+int rep_dd_walkers(int range) {
+    switch (range - kDDRange0) {
+        case 0: return kWave;
+        case 1: return kWave / 2;
+        default: return kWave / 4;
+    }
 }
 
 const void* rep_dd_kernel_fn(int range) {
     switch (range - kDDRange0) {
-        case 0: return reinterpret_cast<const void*>(&rep_dd_kernel<4>);
-        case 1: return reinterpret_cast<const void*>(&rep_dd_kernel<8>);
-        case 2: return reinterpret_cast<const void*>(&rep_dd_kernel<10>);
+        case 0: return reinterpret_cast<const void*>(&rep_dd_kernel<1, 8>);
+        case 1: return reinterpret_cast<const void*>(&rep_dd_kernel<2, 8>);
+        case 2: return reinterpret_cast<const void*>(&rep_dd_kernel<4, 8>);
+        case 3: return reinterpret_cast<const void*>(&rep_dd_kernel<4, 10>);
         default: return nullptr;
     }
 }
 
 hipError_t launch_rep_dd(int range, const RepBatchParams& p, int grid_blocks, hipStream_t st) {
     switch (range - kDDRange0) {
-        case 0: hipLaunchKernelGGL((rep_dd_kernel<4>), dim3(grid_blocks), dim3(256), 0, st, p); break;
-        case 1: hipLaunchKernelGGL((rep_dd_kernel<8>), dim3(grid_blocks), dim3(256), 0, st, p); break;
-        case 2: hipLaunchKernelGGL((rep_dd_kernel<10>), dim3(grid_blocks), dim3(256), 0, st, p); break;
+        case 0: hipLaunchKernelGGL((rep_dd_kernel<1, 8>), dim3(grid_blocks), dim3(256), 0, st, p); break;
+        case 1: hipLaunchKernelGGL((rep_dd_kernel<2, 8>), dim3(grid_blocks), dim3(256), 0, st, p); break;
+        case 2: hipLaunchKernelGGL((rep_dd_kernel<4, 8>), dim3(grid_blocks), dim3(256), 0, st, p); break;
+        case 3: hipLaunchKernelGGL((rep_dd_kernel<4, 10>), dim3(grid_blocks), dim3(256), 0, st, p); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();

@@ -95,10 +95,12 @@ hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks,
 hipError_t launch_rep_reduce(const RepBatchParams& p, hipStream_t st);
 
 // Double-double sweep over the same tasks and items (precision PQB_PREC_INF).
-// Ranges kDDRange0 .. kDDRange0 + 2 cover up to 16, 32 and 40 slots.
+// Ranges kDDRange0 .. kDDRange0 + 3 cover up to 8, 16, 32 and 40 slots with a
+// chunk per 1, 2, 4 and 4 lanes; rep_dd_walkers() = chunks per wave.
 constexpr int kDDRange0 = 5;
-constexpr int kRanges = kDDRange0 + 3;
+constexpr int kRanges = kDDRange0 + 4;
 int rep_dd_range_for(int max_ns);
+int rep_dd_walkers(int range);
 const void* rep_dd_kernel_fn(int range);
 hipError_t launch_rep_dd(int range, const RepBatchParams& p, int grid_blocks, hipStream_t st);
 

@@ -109,6 +109,25 @@ def test_repeated_integer_matrix_is_exact():
         assert got == complex(float(ref), 0.0), (k, got, ref)
 
 
+@pytest.mark.parametrize("mult", [(12, 12, 12), (9, 11, 10, 8)])
+def test_widest_slot_range(mult):
+    """33..40 photons (the widest instantiation, up to 10 slots per lane) with few Gray
+    steps (three or four rows of large multiplicity): equal to the oracle's binary128
+    repeated sweep rounded to double, and the FP64 sweep within 1e-10 of it."""
+    rng = np.random.default_rng(4650 + len(mult))
+    k = len(mult)
+    m = sum(mult)
+    U = random_complex(rng, k, m)
+    rows = np.array(mult)
+    cols = np.ones(m, dtype=np.int64)
+    got = C.permanent_repeated(U, rows, cols, C.ALGO_BBFG, C.PREC_INF)
+    ref = oracle.bbfg_repeated(U, cols, rows, "quad")
+    fp64 = C.permanent_repeated(U, rows, cols, C.ALGO_BBFG, C.PREC_DOUBLE)
+    print(f"photons={m}: got={got!r} quad={ref!r} fp64 rel_err={rel_err(fp64, ref):.2e}", flush=True)
+    assert got == ref
+    assert rel_err(fp64, ref) < 1e-10
+
+
 def test_batch_of_uneven_tasks():
     rng = np.random.default_rng(4700)
     tasks, refs = [], []
