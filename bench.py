@@ -90,6 +90,12 @@ def batched_leg() -> dict:
     C.permanent_batch(tasks)
     t_batch = time.perf_counter() - t0
     replay_ms = C.batch_replay_ms(5) / 5
+    # the same pairs as the unitary plus two (count, modes) state arrays
+    ins = np.stack([i for i, _ in pairs])
+    outs = np.stack([o for _, o in pairs])
+    t0 = time.perf_counter()
+    C.permanent_batch_states(U, ins, outs)
+    t_states = time.perf_counter() - t0
     t0 = time.perf_counter()
     for A, r, c in tasks:
         C.permanent_repeated(A, r, c)
@@ -102,6 +108,7 @@ def batched_leg() -> dict:
         "flops": flops,
         "per_call_s": t_call,
         "batched_s": t_batch,
+        "batched_states_s": t_states,
         "batched_replay_ms": replay_ms,
         "speedup_batched_vs_per_call": t_call / t_batch,
         "speedup_replay_vs_per_call": t_call / (replay_ms * 1e-3),

@@ -159,6 +159,19 @@ typedef struct pqb_task {
 
 int pqb_perm_batch(const pqb_task* tasks, int64_t ntasks, int algo, int precision, double* out);
 
+/* Permanents of `count` (input state, output state) pairs of ONE interferometer U
+ * (rows x cols, row-major complex128, stride ld) in one batched launch: pair i
+ * is the repeated-row permanent of U with row multiplicities
+ * outputs[i*rows .. i*rows+rows) and column multiplicities
+ * inputs[i*cols .. i*cols+cols).  Rows and columns of multiplicity zero drop
+ * out during planning, so the caller builds no per-pair matrix.  Replaces
+ * BatchednPermanentCalculator::calculate (BatchedPermanentCalculator.cpp:
+ * 109-145), which builds the effective matrix of every pair
+ * (adaptInterferometer, :412-451) and runs one calculator per pair; same rules,
+ * messages and precision codes as pqb_perm_batch.  out: 2 * count doubles. */
+int pqb_perm_batch_states(const double* U, int64_t rows, int64_t cols, int64_t ld, const int64_t* inputs,
+                          const int64_t* outputs, int64_t count, int algo, int precision, double* out);
+
 /* pqb_perm_batch with an explicit walk length (Gray steps per chunk) instead
  * of the built-in choice; used by the autotuner.  No reference counterpart. */
 int pqb_perm_batch_tuned(const pqb_task* tasks, int64_t ntasks, int algo, int precision,

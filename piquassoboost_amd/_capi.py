@@ -50,6 +50,8 @@ SIGNATURES = {
     "pqb_perm_repeated": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
                                          ctypes.c_int, ctypes.c_int, _dp]),
     "pqb_perm_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp]),
+    "pqb_perm_batch_states": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
+                                             ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp]),
     "pqb_perm_batch_tuned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int64, _dp]),
     "pqb_batch_replay": (ctypes.c_int, [ctypes.c_int, _dp]),
@@ -225,6 +227,30 @@ def permanent_batch(tasks, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE, 
         _check(lib().pqb_perm_batch(tp, len(tasks), algo, precision, out.ctypes.data_as(_dp)))
     del keep
     return (out[0::2] + 1j * out[1::2]).tolist()
+
+
+def permanent_batch_states(U, inputs, outputs, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE) -> np.ndarray:
+    """Permanents of (input state, output state) pairs of one interferometer U in one
+    launch: inputs is (count, cols), outputs (count, rows); pair i is the repeated-row
+    permanent with row multiplicities outputs[i] and column multiplicities inputs[i]
+    (BatchednPermanentCalculator::calculate without per-pair matrices)."""
+    M = as_matrix(U)
+    ins = np.ascontiguousarray(inputs, dtype=np.int64)
+    outs = np.ascontiguousarray(outputs, dtype=np.int64)
+    if ins.ndim == 1:
+        ins = ins.reshape(1, -1)
+    if outs.ndim == 1:
+        outs = outs.reshape(1, -1)
+    if ins.ndim != 2 or outs.ndim != 2 or ins.shape[0] != outs.shape[0]:
+        raise Exception("inputs and outputs must be (count, modes) arrays with the same count")
+    if ins.shape[1] != M.shape[1] or outs.shape[1] != M.shape[0]:
+        raise Exception("state lengths must match the interferometer: outputs per row, inputs per column")
+    count = ins.shape[0]
+    out = np.zeros(2 * count, dtype=np.float64)
+    _check(lib().pqb_perm_batch_states(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
+                                       ins.ctypes.data_as(_i64p), outs.ctypes.data_as(_i64p), count, algo,
+                                       precision, out.ctypes.data_as(_dp)))
+    return out[0::2] + 1j * out[1::2]
 
 
 def set_walk_lengths(table) -> None:

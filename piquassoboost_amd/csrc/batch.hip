@@ -715,6 +715,24 @@ int pqb_perm_batch_tuned(const pqb_task* tasks, int64_t ntasks, int algo, int pr
     }
 }
 
+int pqb_perm_batch_states(const double* U, int64_t rows, int64_t cols, int64_t ld, const int64_t* inputs,
+                          const int64_t* outputs, int64_t count, int algo, int precision, double* out) {
+    try {
+        if (count < 0 || rows < 0 || cols < 0) return fail(PQB_ERR_ARG, "negative size");
+        if (count == 0) return PQB_OK;
+        if (!out || (rows > 0 && !outputs) || (cols > 0 && !inputs)) return fail(PQB_ERR_ARG, "NULL pointer");
+        if (ld < cols) return fail(PQB_ERR_ARG, "leading dimension smaller than the number of columns");
+        // one task per pair, all pointing into U and the two state arrays
+        std::vector<pqb_task> t((size_t)count);
+        for (int64_t i = 0; i < count; ++i)
+            t[(size_t)i] = pqb_task{U, rows, cols, ld, rows > 0 ? outputs + i * rows : nullptr,
+                                    cols > 0 ? inputs + i * cols : nullptr};
+        return batch_entry(t.data(), count, algo, precision, out, 0);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
 int pqb_set_walk_lengths(const int64_t* walk, int64_t count) {
     std::lock_guard<std::mutex> lk(g_eng.mu);
     for (int i = 0; i <= PQB_MAX_COLS; ++i) g_walk[i] = (walk && i < count && walk[i] > 0) ? walk[i] : 0;

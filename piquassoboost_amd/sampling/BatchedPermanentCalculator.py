@@ -6,8 +6,10 @@ interferometer and computes their permanents in one call -- on the CPU one
 after another (GlynnRep: GlynnPermanentCalculatorRepeatedLongDouble on the
 compact effective matrix, ChinHuh: CChinHuhPermanentCalculator), the batching
 only paying off for the DFE.  Here the accumulated pairs go to the engine as
-one batched launch (pqb_perm_batch) in extended precision for GlynnRep and in
-double for ChinHuh, as the reference's calculators.
+one batched launch (pqb_perm_batch_states: the interferometer once plus the
+state arrays; the engine drops the unoccupied rows and columns itself) in
+extended precision for GlynnRep and in double for ChinHuh, as the reference's
+calculators.
 """
 
 from __future__ import annotations
@@ -55,9 +57,12 @@ class BatchednPermanentCalculator:
         if lib not in (GlynnRep, ChinHuh):
             raise Exception("Wrong value set for permanent library.")
         U = self.interferometer_matrix
-        tasks = [(U, outp, inp) for inp, outp in zip(self.input_states, self.output_states)]
-        if lib == GlynnRep:
-            vals = C.permanent_batch(tasks, algo=C.ALGO_GLYNN, precision=C.PREC_EXTENDED)
-        else:
-            vals = C.permanent_batch(tasks, algo=C.ALGO_BBFG, precision=C.PREC_DOUBLE)
-        return np.asarray(vals, dtype=np.complex128).reshape(1, -1)
+        if not self.input_states:
+            return np.zeros((1, 0), dtype=np.complex128)
+        try:  # one (count, modes) array per side: the engine plans every pair from U itself
+            ins = np.stack([np.asarray(s, dtype=np.int64) for s in self.input_states])
+            outs = np.stack([np.asarray(s, dtype=np.int64) for s in self.output_states])
+        except (ValueError, TypeError):
+            raise Exception("BatchednPermanentCalculator::calculate:  every state needs one entry per mode") from None
+        algo, prec = (C.ALGO_GLYNN, C.PREC_EXTENDED) if lib == GlynnRep else (C.ALGO_BBFG, C.PREC_DOUBLE)
+        return C.permanent_batch_states(U, ins, outs, algo=algo, precision=prec).reshape(1, -1)

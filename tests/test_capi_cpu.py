@@ -239,6 +239,31 @@ def test_minors_edge_and_shape_errors_without_device():
         C.permanent_minors(np.ones((40, 41)))  # 41 columns
 
 
+def test_batch_states_rules_without_device():
+    """pqb_perm_batch_states: shapes, the reference's message for unequal photon counts
+    (with the pair index), and pairs decided by the edge rules need no device."""
+    U = np.eye(4)
+    z = np.zeros((3, 4), dtype=np.int64)
+    assert C.permanent_batch_states(U, z, z).tolist() == [1.0, 1.0, 1.0]
+    with pytest.raises(Exception, match="same count"):
+        C.permanent_batch_states(U, z, z[:2])
+    with pytest.raises(Exception, match="state lengths"):
+        C.permanent_batch_states(U, z[:, :3], z[:, :3])
+    bad_in = np.array([[0, 0, 0, 0], [1, 0, 0, 0]])
+    bad_out = np.array([[0, 0, 0, 0], [2, 0, 0, 0]])
+    with pytest.raises(Exception, match=r"^task 1: BBFGPermanentCalculatorRepeated_Tasks::calculate:  Number of "
+                                        r"input and output states should be equal$"):
+        C.permanent_batch_states(U, bad_in, bad_out)
+    neg = np.array([[-1, 1, 0, 0]])
+    with pytest.raises(C.EngineError):
+        C.permanent_batch_states(U, neg, neg)
+    out = (ctypes.c_double * 2)()
+    L = C.lib()
+    assert L.pqb_perm_batch_states(None, 4, 4, 4, None, None, 1, 0, 0, out) == C.PQB_ERR_ARG
+    assert L.pqb_perm_batch_states(None, 4, 4, 3, None, None, 0, 0, 0, out) == C.PQB_OK  # nothing to do
+    assert L.pqb_perm_batch_states(None, -1, 4, 4, None, None, 1, 0, 0, out) == C.PQB_ERR_ARG
+
+
 def test_batched_front_end_errors_without_device():
     from piquassoboost_amd.sampling.BatchedPermanentCalculator import BatchednPermanentCalculator
 

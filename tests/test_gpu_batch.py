@@ -51,6 +51,22 @@ def test_c4_full_mode_form_matches_compact(c4):
         assert rel_err(full[k], res[k]) < 1e-10
 
 
+def test_c4_states_api_is_bit_identical(c4):
+    """pqb_perm_batch_states (the unitary once, (count, modes) occupation arrays) plans
+    every pair from the full unitary; dropping the unoccupied modes gives the compact
+    matrices of the task list, the same plans and the same batch total, so the 4096
+    values are bit-identical; likewise in double-double."""
+    U, pairs, tasks, res = c4
+    ins = np.stack([p[0] for p in pairs])
+    outs = np.stack([p[1] for p in pairs])
+    got = C.permanent_batch_states(U, ins, outs)
+    assert got.shape == (4096,)
+    assert got.tolist() == res
+    sub = slice(0, 48)
+    dd = C.permanent_batch_states(U, ins[sub], outs[sub], C.ALGO_BBFG, C.PREC_INF)
+    assert dd.tolist() == C.permanent_batch(tasks[sub], C.ALGO_BBFG, C.PREC_INF)
+
+
 def test_c4_per_call_equals_batch(c4):
     U, pairs, tasks, res = c4
     for k in range(0, len(tasks), 97):
