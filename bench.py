@@ -116,6 +116,53 @@ def batched_leg() -> dict:
     }
 
 
+def repeated_leg(reps: int = 10) -> dict:
+    """Config 3: one repeated-row permanent, 30 photons in 60 modes -- inputs in modes
+    0..29, 18 distinct output modes with multiplicities 1 + Multinomial(12, 1/18), seeded
+    60-mode Haar unitary (seed 1004); the compact 18 x 30 matrix.  GPU per call (host API,
+    wall) and per replay of the captured launch; the restated reference sweep (double,
+    std::thread) on the host cores beside it."""
+    from piquassoboost_amd import _capi as C
+    from piquassoboost_amd.sampling.utilities import haar_unitary
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+
+    rng = np.random.default_rng(1004)
+    U = haar_unitary(60, 1004)
+    modes = np.sort(rng.choice(60, size=18, replace=False))
+    rows = (1 + rng.multinomial(12, np.full(18, 1.0 / 18))).astype(np.int64)
+    cols = np.ones(30, dtype=np.int64)
+    A = np.ascontiguousarray(U[np.ix_(modes, np.arange(30))])
+    plan = C.repeated_plan(A, rows, cols, 1 << 40)
+    steps = plan["walk"] * plan["chunks"]
+    flops = steps * (2 * 30 + 6 * 29 + 4)
+    v = C.permanent_repeated(A, rows, cols)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        C.permanent_repeated(A, rows, cols)
+    call_ms = (time.perf_counter() - t0) / reps * 1e3
+    replay_ms = C.batch_replay_ms(reps) / reps
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    h = oracle.bbfg_repeated(A, cols, rows, "double", threads=threads)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    return {
+        "workload": "30 photons in 60 modes, 18 distinct outputs (multiplicities "
+                    + ",".join(str(int(r)) for r in rows) + "), Haar seed 1004",
+        "gray_steps": int(steps),
+        "flops": float(flops),
+        "per_call_ms": call_ms,
+        "replay_ms": replay_ms,
+        "replay_tflops": flops / (replay_ms * 1e-3) / 1e12,
+        "host_reference_ms": host_ms,
+        "host_threads": threads,
+        "gpu_vs_host": host_ms / call_ms,
+        "perm": [v.real, v.imag],
+        "host_rel_diff": abs(h - v) / abs(v),
+    }
+
+
 def grad_leg(reps: int = 10) -> dict:
     """Gradient workload: 256 n=20 and 8 n=26 complex128 blocks, block b the
     top-left corner of the 50-mode Haar unitary of seed 1006 + b, inputs
@@ -293,6 +340,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
+    ap.add_argument("--no-repeated", action="store_true", help="skip the repeated-row (30 photons) leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the forward+backward leg")
     ap.add_argument("--no-sampling", action="store_true", help="skip the boson-sampling leg")
     ap.add_argument("--no-inf", action="store_true", help="skip the double-double (PQB_PREC_INF) leg")
@@ -388,6 +436,8 @@ def main() -> int:
         "event_ms_per_perm": ms_per_step,
         "perm": [perm.real, perm.imag],
     }
+    if not a.no_repeated:
+        line["repeated"] = repeated_leg()
     if not a.no_batch:
         line["batched"] = batched_leg()
     if not a.no_grad:

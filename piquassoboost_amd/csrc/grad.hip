@@ -34,7 +34,7 @@ struct GradPlan {
 // Register bits per column count: the largest K whose accumulators and row
 // data (4 M doubles each) still fit next to the column sums, running sums and
 // suffix products in 256 VGPRs (the instantiations of perm_grad.hip).
-static int grad_register_bits(int M) { return M <= 5 ? 3 : (M <= 7 ? 2 : 1); }
+static int grad_register_bits(int M) { return M <= 8 ? 3 : 2; }
 
 // Staged matrix (n x 4M) plus, per wave, the 16 x 4M transpose area and (adjoint
 // mode) the H x 4M high-row accumulators.

@@ -28,7 +28,8 @@
 // * Sign bits (bit b <-> row n-1-b): bits [0, K) are "register rows" walked
 //   in time inside a block of 2^K steps (a fully unrolled reflected Gray walk,
 //   so the flipped row is a compile-time index and its G accumulators live in
-//   registers); bits [K, K+L) are the delta-lane index (constant per lane);
+//   registers; its data too up to M = 5, above that it is read from the staged
+//   matrix in LDS); bits [K, K+L) are the delta-lane index (constant per lane);
 //   bits [K+L, n-1) are "high rows", flipped in time between blocks.  The time
 //   index tau = block * 2^K + t runs a reflected Gray code over the K register
 //   and H high bits; a wave owns a contiguous run of blocks.
@@ -122,10 +123,19 @@ __device__ __forceinline__ void grad_term(const double (&cr)[M], const double (&
 // of T over all lanes is, per column j, 2^(R-1) perm(A without column j).  No
 // row is differentiated, so flips update the column sums only.
 template <int M, int K, bool MINORS>
-__global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void grad_sweep_kernel(GradParams p) {
     extern __shared__ double2 lds[];
     constexpr int NP = 4 * M;   // padded column count
     constexpr int KA = K > 0 ? K : 1;
+    // Register budget (two waves per SIMD): up to M = 5 the register rows' data
+    // sits in VGPRs; from M = 6 it is read from the staged matrix at each flip,
+    // and the scheduler is kept from interleaving steps across blocks (M = 6, 7)
+    // or across steps (M >= 8).  Measured on n = 26 (M = 7, K = 3): 4.09 ms for
+    // 8 matrices against 4.32 ms with K = 2 and VGPR rows; on n = 20 the VGPR
+    // rows are faster (1.59 ms against 1.64-1.70 ms for 256 matrices).
+    constexpr bool kRowsInLds = M >= 6;
+    constexpr int kSchedEvery = M >= 8 ? 1 : (1 << K);  // steps between scheduling barriers
     const int n = p.n, R = p.n, L = p.L, H = p.H, NCOL = p.cols;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -197,15 +207,17 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
     // lane sign = s_time * (-1)^popcount(d): s_time from the time rows (wave-uniform)
     const double s_time = (__popcll((unsigned long long)gam0) & 1) ? -1.0 : 1.0;
 
-    // register rows' data (row n-1-b for bit b < K)
-    double arr[KA][M], ari[KA][M];
+    // register rows' data (row n-1-b for bit b < K) when held in VGPRs
+    double arr[kRowsInLds ? 1 : KA][M], ari[kRowsInLds ? 1 : KA][M];
+    if constexpr (!kRowsInLds) {
 #pragma unroll
-    for (int b = 0; b < KA; ++b) {
+        for (int b = 0; b < KA; ++b) {
 #pragma unroll
-        for (int k = 0; k < M; ++k) {
-            const double2 a = (b < K) ? As[(R - 1 - b) * NP + q * M + k] : make_double2(0.0, 0.0);
-            arr[b][k] = a.x;
-            ari[b][k] = a.y;
+            for (int k = 0; k < M; ++k) {
+                const double2 a = (b < K) ? As[(R - 1 - b) * NP + q * M + k] : make_double2(0.0, 0.0);
+                arr[b][k] = a.x;
+                ari[b][k] = a.y;
+            }
         }
     }
 
@@ -229,6 +241,11 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
     constexpr int kPer = kPairs ? 8 : 16;
 
     for (int64_t g = g0; g < g1; ++g) {
+        // register rows read from the staged matrix at each flip: the offset is
+        // made opaque per block so that the loads are not hoisted out of the
+        // walk (their values would then occupy VGPRs throughout)
+        int rbase = (R - 1) * NP + q * M;
+        if constexpr (kRowsInLds) asm volatile("" : "+v"(rbase));
         if (g > g0) {
             // ---- high-row flip: time bit K + h, h = ctz(g); new delta = +1 iff bit h+1 of g
             const int h = __builtin_ctzll((unsigned long long)g);
@@ -283,6 +300,7 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
         // with K = 0 every block is one step and the parity alternates
         if (K == 0 && ((g - g0) & 1)) grad_term<M, true>(cr, ci, tr, ti);
         else grad_term<M, false>(cr, ci, tr, ti);
+        if constexpr (kSchedEvery == 1) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int t = 1; t < (1 << K); ++t) {
             // ---- register-row flip: bit b = ctz(t); new delta = +1 iff bit b+1 of tau
@@ -295,11 +313,20 @@ __global__ __launch_bounds__(256) void grad_sweep_kernel(GradParams p) {
                     zr[b][k] = fma(-f, tr[k], zr[b][k]);
                     zi[b][k] = fma(-f, ti[k], zi[b][k]);
                 }
-                cr[k] = fma(f, arr[b][k], cr[k]);
-                ci[k] = fma(f, ari[b][k], ci[k]);
+                if constexpr (kRowsInLds) {
+                    const double2 a = As[rbase - b * NP + k];
+                    cr[k] = fma(f, a.x, cr[k]);
+                    ci[k] = fma(f, a.y, ci[k]);
+                } else {
+                    cr[k] = fma(f, arr[b][k], cr[k]);
+                    ci[k] = fma(f, ari[b][k], ci[k]);
+                }
             }
             if (t & 1) grad_term<M, true>(cr, ci, tr, ti);
             else grad_term<M, false>(cr, ci, tr, ti);
+            if constexpr (kRowsInLds) {
+                if ((t + 1) % kSchedEvery == 0) __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
 
@@ -451,7 +478,7 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(GradReduceParams p) {
 // ------------------------------------------------------------ dispatch --
 
 #define PQB_GRAD_CASES(X) \
-    X(1, 0) X(2, 0) X(2, 1) X(2, 2) X(2, 3) X(3, 3) X(4, 3) X(5, 3) X(6, 2) X(7, 2) X(8, 1) X(9, 1) X(10, 1)
+    X(1, 0) X(2, 0) X(2, 1) X(2, 2) X(2, 3) X(3, 3) X(4, 3) X(5, 3) X(6, 3) X(7, 3) X(8, 3) X(9, 2) X(10, 2)
 
 const void* grad_kernel_fn(int M, int K, bool minors) {
 #define PQB_CASE(MM, KK)                                                                        \

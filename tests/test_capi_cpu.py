@@ -198,8 +198,8 @@ def test_repeated_plan_host_only():
     assert p2["walk"] * p2["chunks"] == 3 and p2["slots"] == 3 and p2["rows"] == 2
 
 
-GRAD_INSTANTIATIONS = {(1, 0), (2, 0), (2, 1), (2, 2), (2, 3), (3, 3), (4, 3), (5, 3), (6, 2), (7, 2), (8, 1),
-                       (9, 1), (10, 1)}
+GRAD_INSTANTIATIONS = {(1, 0), (2, 0), (2, 1), (2, 2), (2, 3), (3, 3), (4, 3), (5, 3), (6, 3), (7, 3), (8, 3),
+                       (9, 2), (10, 2)}
 
 
 def test_grad_plan_host_only():
