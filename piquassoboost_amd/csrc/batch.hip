@@ -405,7 +405,10 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     std::vector<RepItem> items;
     std::vector<ItemCost> costs;
     std::vector<std::pair<uint32_t, uint32_t>> task_items(live.size());
-    const uint64_t lane_steps = 2048;     // target Gray steps per lane (quad) per item
+    // target Gray steps per lane (quad) per item: small items balance the tail of
+    // an uneven batch (config-4 replay 9.17 / 8.81 / 8.68 / 8.64 ms at 2048 / 1024 /
+    // 512 / 256 steps), the cost being one atomic fetch per item
+    const uint64_t lane_steps = 512;
     const uint64_t walkers = dd ? (uint64_t)rep_dd_walkers(range) : 64;  // chunks a wave works on at once
     auto per_item_of = [&](const TaskPlan& P) { return walkers * std::max<uint64_t>(1, lane_steps / P.L); };
     // A batch too small to give every resident wave ~4 items at that size
