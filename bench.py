@@ -69,7 +69,7 @@ def cpu_baseline(A: np.ndarray, budget_s: float = 20.0) -> dict:
 
 
 def batched_leg() -> dict:
-    """Config 4: 4096 uneven repeated-row permanents (n in [8, 26], output
+    """4096 uneven repeated-row permanents (n in [8, 26], output
     multiplicities 1/2/3 with p = .7/.2/.1) from one 100-mode Haar unitary, as
     one batched launch against 4096 per-call launches on the same GPU."""
     from piquassoboost_amd import _capi as C
@@ -117,7 +117,7 @@ def batched_leg() -> dict:
 
 
 def repeated_leg(reps: int = 10) -> dict:
-    """Config 3: one repeated-row permanent, 30 photons in 60 modes -- inputs in modes
+    """One repeated-row permanent, 30 photons in 60 modes -- inputs in modes
     0..29, 18 distinct output modes with multiplicities 1 + Multinomial(12, 1/18), seeded
     60-mode Haar unitary (seed 1004); the compact 18 x 30 matrix.  GPU per call (host API,
     wall) and per replay of the captured launch; the restated reference sweep (double,

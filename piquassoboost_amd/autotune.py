@@ -8,7 +8,7 @@ sub-batch for a list of candidate walk lengths, and keeps the fastest per
 photon count.  The table is stored as JSON next to this module and applied
 when the engine library is loaded.
 
-    python -m piquassoboost_amd.autotune [--out PATH]   # tunes on the config-4 batch
+    python -m piquassoboost_amd.autotune [--out PATH]   # tunes on the 4096-task uneven batch
 """
 
 from __future__ import annotations
@@ -66,7 +66,7 @@ def main() -> int:
     out.parent.mkdir(parents=True, exist_ok=True)
     out.write_text(json.dumps({
         "device": "gfx950",
-        "workload": "4096 uneven repeated-row permanents (config 4), replay ms per photon-count group",
+        "workload": "4096 uneven repeated-row permanents (n ~ U[8,26]), replay ms per photon-count group",
         "walk_length_by_photons": {str(k): v for k, v in best.items()},
         "replay_ms": {str(k): {str(L): round(t, 5) for L, t in v.items()} for k, v in timings.items()},
     }, indent=1))

@@ -406,7 +406,7 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     std::vector<ItemCost> costs;
     std::vector<std::pair<uint32_t, uint32_t>> task_items(live.size());
     // target Gray steps per lane (quad) per item: small items balance the tail of
-    // an uneven batch (config-4 replay 9.17 / 8.81 / 8.68 / 8.64 ms at 2048 / 1024 /
+    // an uneven batch (4096 tasks of 8..26 photons: replay 9.17 / 8.81 / 8.68 / 8.64 ms at 2048 / 1024 /
     // 512 / 256 steps), the cost being one atomic fetch per item
     const uint64_t lane_steps = 512;
     const uint64_t walkers = dd ? (uint64_t)rep_dd_walkers(range) : 64;  // chunks a wave works on at once

@@ -64,7 +64,7 @@ constexpr int kQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
 constexpr int kQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
 // (The same permutation through ds_swizzle moves the 8 moves per step to the
 // LDS pipe but puts its latency on the step's critical path: measured 4% slower
-// on the C5 workload at 2 waves per SIMD.)
+// on 256 n=20 + 8 n=26 blocks at 2 waves per SIMD.)
 
 // LDS accesses of one wave to locations written by other lanes of the same
 // wave: LDS executes a wave's instructions in order, the fences keep the

@@ -92,7 +92,7 @@ def test_clements_chain_matches_torch_ryser():
     assert (grads[0] - grads[1]).abs().max() <= 1e-11 * grads[0].abs().max()
 
 
-def test_c5_clements_chain_finite_differences():
+def test_clements_chain_finite_differences():
     """Gradient workload blocks b = 0 (n = 20) and b = 256 (n = 26): the
     50-mode Haar unitaries of seeds 1006 and 1262, Clements-decomposed and
     rebuilt on the GPU; loss = sum log|perm(top-left block)|^2; autograd
@@ -115,7 +115,7 @@ def test_c5_clements_chain_finite_differences():
         for (n, _, B), M, (t, p, a) in zip(cases, meshes, params):
             assert (unitary_torch(M.modes, t, p, a)[:n, :n].cpu() - torch.tensor(B)).abs().max() < 1e-12
     loss = loss_of(params)
-    print(f"C5 chain: loss={loss.item():.6f}", flush=True)
+    print(f"autograd chain: loss={loss.item():.6f}", flush=True)
     loss.backward()
     rng = np.random.default_rng(55)
     h = 1e-6
@@ -138,4 +138,4 @@ def test_c5_clements_chain_finite_differences():
                 err = abs(fd - g) / max(abs(g), 1.0)
                 worst = max(worst, err)
                 assert err < 1e-5, (m, which, int(k), fd, g)
-    print(f"C5 chain finite differences: worst {worst:.3e}", flush=True)
+    print(f"autograd chain finite differences: worst {worst:.3e}", flush=True)

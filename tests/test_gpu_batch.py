@@ -1,4 +1,4 @@
-"""Config 4: 4096 uneven repeated-row permanents (n in [8, 26], output
+"""4096 uneven repeated-row permanents (n in [8, 26], output
 multiplicities in {1, 2, 3}) from one seeded 100-mode Haar unitary, computed
 in one batched launch; results checked against the restated reference (dd)
 to 1e-10 and against the per-call path."""
@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def c4():
+def uneven_batch4096():
     U, pairs = uneven_batch(seed=1005, count=4096)
     tasks = []
     for inp, outp in pairs:
@@ -25,8 +25,8 @@ def c4():
     return U, pairs, tasks, res
 
 
-def test_c4_against_oracle(c4):
-    U, pairs, tasks, res = c4
+def test_uneven_batch_against_oracle(uneven_batch4096):
+    U, pairs, tasks, res = uneven_batch4096
     rng = np.random.default_rng(7)
     pick = rng.choice(len(tasks), size=40, replace=False)
     worst = 0.0
@@ -36,27 +36,27 @@ def test_c4_against_oracle(c4):
         err = rel_err(res[k], ref)
         worst = max(worst, err)
         assert err < 1e-10, (int(k), int(cols.sum()), err)
-    print(f"C4 sample of 40: worst rel_err={worst:.3e}", flush=True)
+    print(f"uneven batch, sample of 40: worst rel_err={worst:.3e}", flush=True)
 
 
-def test_c4_full_mode_form_matches_compact(c4):
+def test_uneven_batch_full_mode_form_matches_compact(uneven_batch4096):
     """The same transitions given on the full 100-mode unitary with zero
     multiplicities (what BBFGRepeatedPermanentCalculator receives from a
     sampler) give the same values.  The walk length depends on the batch's
     total work, so a 256-task batch is chunked differently from the
     4096-task one: agreement is to the parity bar, not bit for bit."""
-    U, pairs, tasks, res = c4
+    U, pairs, tasks, res = uneven_batch4096
     full = C.permanent_batch([(U, outp, inp) for inp, outp in pairs[:256]])
     for k in range(256):
         assert rel_err(full[k], res[k]) < 1e-10
 
 
-def test_c4_states_api_is_bit_identical(c4):
+def test_uneven_batch_states_api_is_bit_identical(uneven_batch4096):
     """pqb_perm_batch_states (the unitary once, (count, modes) occupation arrays) plans
     every pair from the full unitary; dropping the unoccupied modes gives the compact
     matrices of the task list, the same plans and the same batch total, so the 4096
     values are bit-identical; likewise in double-double."""
-    U, pairs, tasks, res = c4
+    U, pairs, tasks, res = uneven_batch4096
     ins = np.stack([p[0] for p in pairs])
     outs = np.stack([p[1] for p in pairs])
     got = C.permanent_batch_states(U, ins, outs)
@@ -67,28 +67,28 @@ def test_c4_states_api_is_bit_identical(c4):
     assert dd.tolist() == C.permanent_batch(tasks[sub], C.ALGO_BBFG, C.PREC_INF)
 
 
-def test_c4_per_call_equals_batch(c4):
-    U, pairs, tasks, res = c4
+def test_uneven_batch_per_call_equals_batch(uneven_batch4096):
+    U, pairs, tasks, res = uneven_batch4096
     for k in range(0, len(tasks), 97):
         A, rows, cols = tasks[k]
         assert rel_err(C.permanent_repeated(A, rows, cols), res[k]) < 1e-10
 
 
-def test_c4_replay_deterministic(c4):
+def test_uneven_batch_replay_deterministic(uneven_batch4096):
     """A graph replay recomputes the resident batch; the launch is
     deterministic (fixed item order per task in the reduction)."""
-    U, pairs, tasks, res = c4
+    U, pairs, tasks, res = uneven_batch4096
     res2 = C.permanent_batch(tasks)
     assert res2 == res
     assert C.batch_replay_ms(2) > 0.0
 
 
-def test_batched_front_end(c4):
+def test_batched_front_end(uneven_batch4096):
     """BatchednPermanentCalculator (one interferometer, accumulated state pairs) on 64 of
     the uneven transitions: both lib codes against the per-call path and the oracle."""
     from piquassoboost_amd.sampling.BatchedPermanentCalculator import BatchednPermanentCalculator
 
-    U, pairs, tasks, res = c4
+    U, pairs, tasks, res = uneven_batch4096
     b = BatchednPermanentCalculator(U)
     b.reserve_space(64)
     for k in range(64):

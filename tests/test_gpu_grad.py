@@ -36,7 +36,7 @@ def perm_oracle(A, prec="double"):
 
 
 @pytest.fixture(scope="module")
-def c5():
+def grad_blocks():
     """256 n=20 and 8 n=26 top-left blocks of the Haar unitaries of seeds 1006 + b."""
     blocks, seeds = gradient_blocks()
     return blocks
@@ -66,25 +66,25 @@ def test_real_integer_matrix_exact():
     assert np.array_equal(G.real, ref) and not np.any(G.imag)
 
 
-def test_c5_n20_all_minors(c5):
-    blocks = c5
+def test_grad_n20_all_minors(grad_blocks):
+    blocks = grad_blocks
     A_all = blocks[20][:2]
     perms, G = C.permanent_grad_batch(A_all)
     worst = 0.0
     for b in range(2):
         A = A_all[b]
-        print(f"C5 n=20 block {b}: 400 minors", flush=True)
+        print(f"grad n=20 block {b}: 400 minors", flush=True)
         for i in range(20):
             for j in range(20):
                 e = rel_err(G[b, i, j], perm_oracle(minor(A, i, j)))
                 worst = max(worst, e)
                 assert e < 1e-9, (b, i, j, e)
         assert rel_err(perms[b], perm_oracle(A, "dd")) < 1e-12
-    print(f"C5 n=20 minors: worst rel_err={worst:.3e}", flush=True)
+    print(f"grad n=20 minors: worst rel_err={worst:.3e}", flush=True)
 
 
-def test_c5_n26_sampled_minors(c5):
-    blocks = c5
+def test_grad_n26_sampled_minors(grad_blocks):
+    blocks = grad_blocks
     A = blocks[26][0]
     p, G = C.permanent_grad(A)
     rng = np.random.default_rng(5)
@@ -95,16 +95,16 @@ def test_c5_n26_sampled_minors(c5):
         worst = max(worst, e)
         assert e < 1e-9, (i, j, e)
     assert rel_err(p, perm_oracle(A, "dd")) < 1e-10
-    print(f"C5 n=26 sampled minors: worst rel_err={worst:.3e}", flush=True)
+    print(f"grad n=26 sampled minors: worst rel_err={worst:.3e}", flush=True)
 
 
-def test_c5_finite_differences(c5):
+def test_grad_finite_differences(grad_blocks):
     """64 entries of the first n=20 block picked with seed 1007, h = 1e-6: perm
     is linear in each entry, so the central difference of perm equals G_ij up
     to the rounding of the two permanents (double-double oracle); the loss
     L = log|perm|^2 has dL/dRe a = 2 Re(conj(p) G)/|p|^2 and
     dL/dIm a = -2 Im(conj(p) G)/|p|^2."""
-    blocks = c5
+    blocks = grad_blocks
     A = blocks[20][0]
     p, G = C.permanent_grad(A)
     rng = np.random.default_rng(1007)
@@ -124,7 +124,7 @@ def test_c5_finite_differences(c5):
         fdl = (Lp - Lm) / (2 * h)
         anl = (-2 * (np.conj(p) * G[i, j]).imag if imag else 2 * (np.conj(p) * G[i, j]).real) / abs(p) ** 2
         worst_l = max(worst_l, abs(fdl - anl) / max(abs(anl), 1e-3))
-    print(f"C5 finite differences: perm worst rel {worst_p:.3e}, loss worst {worst_l:.3e}", flush=True)
+    print(f"grad finite differences: perm worst rel {worst_p:.3e}, loss worst {worst_l:.3e}", flush=True)
     assert worst_p < 1e-8
     assert worst_l < 1e-6
 
@@ -141,12 +141,12 @@ def laplace_check(A, p, G, rows, rng, tol):
         assert rel_err(np.dot(b, G[i]), C.permanent(B)) < tol, ("replaced row", i)
 
 
-def test_c5_full_batch_laplace(c5):
+def test_grad_full_batch_laplace(grad_blocks):
     """The whole gradient workload (256 n=20 and 8 n=26 blocks): perms against
     the plain sweep and the Laplace expansion along row 0 and column 0."""
-    blocks = c5
+    blocks = grad_blocks
     for n, B in sorted(blocks.items()):
-        print(f"C5 batch n={n} x {B.shape[0]}", flush=True)
+        print(f"grad batch n={n} x {B.shape[0]}", flush=True)
         perms, G = C.permanent_grad_batch(B)
         fwd = C.permanent_batch([(B[b], np.ones(n, np.int64), np.ones(n, np.int64)) for b in range(B.shape[0])])
         for b in range(B.shape[0]):

@@ -156,7 +156,7 @@ def test_haar_blocks_match_the_binary128_oracle(n):
 
 
 @pytest.mark.parametrize("name", ["c2a_n28", "c2a_n30"])
-def test_c2a_matches_the_double_double_fixture(haar_blocks, name):
+def test_haar_n28_n30_match_the_double_double_fixture(haar_blocks, name):
     A = haar_blocks[name]
     h = haar_blocks[name + "_dd"]
     ref = complex(h[0] + h[1], h[2] + h[3])

@@ -12,8 +12,8 @@ from piquassoboost_amd.sampling import Boson_Sampling_Utilities as BSU
 
 pytestmark = pytest.mark.gpu
 
-# The FP64 sweep is compared with the double-double oracle.  Tolerances: the
-# north star's 1e-12 (C1, n=20) and 1e-10 (C2a, n=28/30) relative errors; the
+# The FP64 sweep is compared with the double-double oracle.  Tolerances: 1e-12
+# relative error for the n=20 Haar block and 1e-10 for the n=28/30 blocks; the
 # small random cases use 1e-12 as well.
 TOL_SMALL = 1e-12
 
@@ -71,21 +71,21 @@ def test_list_batched_plain():
         assert rel_err(v, oracle.bbfg(m, "dd")) < TOL_SMALL
 
 
-def test_c1_haar_n20(haar_blocks):
-    """Config 1: n=20 block of a 40-mode Haar unitary, 1e-12 vs the reference
+def test_haar_n20(haar_blocks):
+    """n=20 block of a 40-mode Haar unitary, 1e-12 vs the reference
     GlynnPermanentCalculator (long double) restated."""
     A = haar_blocks["c1_n20"]
     ref = complex(*haar_blocks["c1_n20_glynn_ld"])
     for lib_cls in (BSU.GlynnPermanent, BSU.BBFGPermanentDouble):
         got = lib_cls(A).calculate()
         err = rel_err(got, ref)
-        print(f"C1 {lib_cls.__name__}: rel_err={err:.3e}", flush=True)
+        print(f"n=20 Haar {lib_cls.__name__}: rel_err={err:.3e}", flush=True)
         assert err < 1e-12
 
 
 @pytest.mark.parametrize("name", ["c2a_n28", "c2a_n30"])
-def test_c2a_haar_large(haar_blocks, name):
-    """Config 2a: n=28 / n=30 blocks of a 60-mode Haar unitary within 1e-10 of
+def test_haar_n28_n30(haar_blocks, name):
+    """n=28 / n=30 blocks of a 60-mode Haar unitary within 1e-10 of
     the double-double oracle."""
     A = haar_blocks[name]
     h = haar_blocks[name + "_dd"]
@@ -140,8 +140,8 @@ def test_fp64_path_bit_exact_on_01_below_2p53():
         assert got.real == float(exact) and got.imag == 0.0, (n, got, exact)
 
 
-def test_c2b_bernoulli26_exact():
-    """Config 2b: n=26 Bernoulli(1/2) 0/1 matrices; the exact 128-bit result
+def test_bernoulli26_exact():
+    """n=26 Bernoulli(1/2) 0/1 matrices; the exact 128-bit result
     equals the oracle's integer permanent, and the double result is its
     correctly rounded value."""
     d = np.load(GOLDEN / "bernoulli26.npz")

@@ -20,14 +20,14 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def test_c3_golden():
-    """Config 3: 30 photons, 60 modes, 18 occupied output modes."""
+def test_30_photons_18_outputs_golden():
+    """30 photons, 60 modes, 18 occupied output modes."""
     d = np.load(GOLDEN / "repeated_c3.npz")
     h = d["value_dd"]
     ref = complex(h[0] + h[1], h[2] + h[3])
     got = C.permanent_repeated(d["A"], d["row_mult"], d["col_mult"])
     err = rel_err(got, ref)
-    print(f"C3 compact: rel_err={err:.3e}", flush=True)
+    print(f"30 photons / 18 outputs, compact: rel_err={err:.3e}", flush=True)
     assert err < 1e-10
     # the same transition on the full 60-mode unitary with zero multiplicities
     full = BSU.BBFGRepeatedPermanentCalculatorDouble(d["U"], d["input_state"], d["output_state"]).calculate()

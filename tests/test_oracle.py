@@ -175,7 +175,7 @@ def test_golden_haar_consistency(haar_blocks):
         h = haar_blocks[name + "_dd"]
         dd = complex(h[0] + h[1], h[2] + h[3])
         assert rel_err(ld, dd) < 1e-13, name
-    # C1 is quoted against the reference GlynnPermanentCalculator (long double)
+    # the n=20 Haar block is quoted against the reference GlynnPermanentCalculator (long double)
     gr = complex(*haar_blocks["c1_n20_glynn_ld"])
     assert rel_err(gr, complex(*haar_blocks["c1_n20_ld"])) < 1e-13
 
