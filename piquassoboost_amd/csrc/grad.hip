@@ -33,8 +33,12 @@ struct GradPlan {
 
 // Register bits per column count: the largest K whose accumulators and row
 // data (4 M doubles each) still fit next to the column sums, running sums and
-// suffix products in 256 VGPRs (the instantiations of perm_grad.hip).
-static int grad_register_bits(int M) { return M <= 8 ? 3 : 2; }
+// suffix products in 256 VGPRs (the instantiations of perm_grad.hip); K = 4
+// where it was measured faster than K = 3 (n = 17..20 and 25..28).
+static int grad_register_bits(int M) {
+    if (M == 5 || M == 7) return 4;
+    return M <= 8 ? 3 : 2;
+}
 
 // Staged matrix (n x 4M) plus, per wave, the 16 x 4M transpose area and (adjoint
 // mode) the H x 4M high-row accumulators.

@@ -78,7 +78,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 // One Gray step's contribution: y_j = (-1)^t * prod_{l != j} c_l for the
 // lane's M columns, accumulated into T.  (The permanent itself is not
 // accumulated: it is sum_j a_0j G_0j of the result, grad_perm_kernel.)
-template <int M, bool ODD>
+// PIN: end the step with T in registers.  T is not read again within the step,
+// so instruction selection may emit its updates late (after the next flip's
+// loads), which keeps the prefix and suffix products of the step live until
+// then; an empty asm statement that takes and returns T closes the step.
+template <int M, bool ODD, bool PIN>
 __device__ __forceinline__ void grad_term(const double (&cr)[M], const double (&ci)[M], double (&tr)[M],
                                           double (&ti)[M]) {
     // suffix products s_k = c_k ... c_{M-1}; s_0 is this quarter's product
@@ -115,6 +119,10 @@ __device__ __forceinline__ void grad_term(const double (&cr)[M], const double (&
             ti[k] += pi;
         }
     }
+    if constexpr (PIN) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) asm volatile("" : "+v"(tr[k]), "+v"(ti[k]));
+    }
 }
 
 // MINORS = false: the adjoint sweep of square n x n matrices (n = p.n = p.cols).
@@ -133,9 +141,14 @@ void grad_sweep_kernel(GradParams p) {
     // and the scheduler is kept from interleaving steps across blocks (M = 6, 7)
     // or across steps (M >= 8).  Measured on n = 26 (M = 7, K = 3): 4.09 ms for
     // 8 matrices against 4.32 ms with K = 2 and VGPR rows; on n = 20 the VGPR
-    // rows are faster (1.59 ms against 1.64-1.70 ms for 256 matrices).
+    // rows are faster (1.59 ms against 1.64-1.70 ms for 256 matrices).  K = 4
+    // (M = 5, 7: one high-row flip per 16 steps) fits in 256 VGPRs only with the
+    // running sums and accumulators pinned at the end of each step (without, the
+    // M = 7 build spills); measured 1.46 / 4.20 ms against 1.53 / 4.27 ms for
+    // K = 3.  Pinning at K = 3 costs 3% on n = 26.
     constexpr bool kRowsInLds = M >= 6;
     constexpr int kSchedEvery = M >= 8 ? 1 : (1 << K);  // steps between scheduling barriers
+    constexpr bool kPin = K >= 4;
     const int n = p.n, R = p.n, L = p.L, H = p.H, NCOL = p.cols;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -298,8 +311,8 @@ void grad_sweep_kernel(GradParams p) {
         }
         // step sign (-1)^(tau - tau0): blocks of 2^K >= 2 steps start even;
         // with K = 0 every block is one step and the parity alternates
-        if (K == 0 && ((g - g0) & 1)) grad_term<M, true>(cr, ci, tr, ti);
-        else grad_term<M, false>(cr, ci, tr, ti);
+        if (K == 0 && ((g - g0) & 1)) grad_term<M, true, kPin>(cr, ci, tr, ti);
+        else grad_term<M, false, kPin>(cr, ci, tr, ti);
         if constexpr (kSchedEvery == 1) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int t = 1; t < (1 << K); ++t) {
@@ -322,8 +335,12 @@ void grad_sweep_kernel(GradParams p) {
                     ci[k] = fma(f, ari[b][k], ci[k]);
                 }
             }
-            if (t & 1) grad_term<M, true>(cr, ci, tr, ti);
-            else grad_term<M, false>(cr, ci, tr, ti);
+            if constexpr (kPin && !MINORS) {  // the accumulators too (see grad_term)
+#pragma unroll
+                for (int k = 0; k < M; ++k) asm volatile("" : "+v"(zr[b][k]), "+v"(zi[b][k]));
+            }
+            if (t & 1) grad_term<M, true, kPin>(cr, ci, tr, ti);
+            else grad_term<M, false, kPin>(cr, ci, tr, ti);
             if constexpr (kRowsInLds) {
                 if ((t + 1) % kSchedEvery == 0) __builtin_amdgcn_sched_barrier(0);
             }
@@ -423,7 +440,9 @@ void grad_sweep_kernel(GradParams p) {
                 *a = make_double2(vr, vi);
             };
             put(0, tbr, tbi);  // row 0
-            for (int b = 0; b < L; ++b) put(R - 1 - (K + b), lr[b], li[b]);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)  // static indices keep lr, li in registers
+                if (b < L) put(R - 1 - (K + b), lr[b], li[b]);
 #pragma unroll
             for (int b = 0; b < K; ++b) {
                 const double s = ((gam_end >> b) & 1) ? -1.0 : 1.0;
@@ -478,7 +497,8 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(GradReduceParams p) {
 // ------------------------------------------------------------ dispatch --
 
 #define PQB_GRAD_CASES(X) \
-    X(1, 0) X(2, 0) X(2, 1) X(2, 2) X(2, 3) X(3, 3) X(4, 3) X(5, 3) X(6, 3) X(7, 3) X(8, 3) X(9, 2) X(10, 2)
+    X(1, 0) X(2, 0) X(2, 1) X(2, 2) X(2, 3) X(3, 3) X(4, 3) X(5, 3) X(6, 3) X(7, 3) X(8, 3) X(9, 2) X(10, 2) \
+    X(5, 4) X(7, 4)
 
 const void* grad_kernel_fn(int M, int K, bool minors) {
 #define PQB_CASE(MM, KK)                                                                        \

@@ -199,7 +199,7 @@ def test_repeated_plan_host_only():
 
 
 GRAD_INSTANTIATIONS = {(1, 0), (2, 0), (2, 1), (2, 2), (2, 3), (3, 3), (4, 3), (5, 3), (6, 3), (7, 3), (8, 3),
-                       (9, 2), (10, 2)}
+                       (9, 2), (10, 2), (5, 4), (7, 4)}
 
 
 def test_grad_plan_host_only():
