@@ -144,14 +144,14 @@ static int ceil_log2(int64_t v) {
 }
 
 // Split of the R-1 sign bits into walk (K), lane (B) and group (H) bits and
-// the number of waves.  Large sweeps launch the smallest power of two of
-// waves >= the resident waves with two groups each: every wave has the same
-// work, and when the resident count is not a power of two (3 waves per SIMD)
-// the surplus waves run as a short second round at low occupancy, where a
-// wave runs faster.  Measured on MI355X at n=30 (profiles/): 2.97 ms against
-// 3.06-3.13 ms for one group per wave at 2 waves per SIMD or ~43 groups per
-// wave at 3.  Small sweeps take the shortest walk that still amortises the
-// seed.
+// the number of waves.  The kernel gives every wave one contiguous run of the
+// walk index (group * 2^K + t) in step pairs, so runs are equal to within one
+// pair and may start and end inside a group: the launch is exactly the
+// resident waves, one round at full occupancy.  (Whole groups per wave would
+// need a power-of-two wave count: 4096 waves of two groups each at n=30 left a
+// quarter of the work to a second round at one wave per SIMD.)  The walk is
+// long enough that a run spans a few groups; small sweeps take the shortest
+// walk that still amortises the seed.
 PlainPlan make_plain_plan(int R, int NC, int64_t waves_resident) {
     (void)NC;
     PlainPlan pl;
@@ -163,8 +163,8 @@ PlainPlan make_plain_plan(int R, int NC, int64_t waves_resident) {
     pl.K = K;
     pl.B = std::min(6, T - K);
     pl.H = T - K - pl.B;
-    const int64_t G = (int64_t)1 << pl.H;
-    int64_t waves = std::min<int64_t>(G, (int64_t)1 << ceil_log2(wr));
+    const int64_t units = ((int64_t)1 << (pl.H + pl.K)) / (pl.K >= 1 ? 2 : 1);  // step pairs
+    int64_t waves = std::min<int64_t>(units, wr);
     if (waves >= 4) {
         waves -= waves % 4;
         pl.block = kBlock;

@@ -124,7 +124,11 @@ void plain_sweep_kernel(PlainParams p) {
     double* const out = p.out + 6 * (size_t)task;
 
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    // the wave index as a scalar: the run bounds, walk positions and row offsets
+    // derived from it are then uniform for the compiler too (scalar loads, SGPR
+    // operands), not only in fact
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / kWave) +
+                         __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
 
     const int R = p.R;
@@ -140,17 +144,24 @@ void plain_sweep_kernel(PlainParams p) {
     const unsigned W = 1u << K;
     const bool lane_on = lane < (1 << B);
 
-    // contiguous run of groups [g0, g1) for this wave (sizes differ by at most one)
-    const int64_t G = (int64_t)1 << H;
-    const int64_t g0 = (G * wave) / nwaves;
-    const int64_t g1 = (G * (wave + 1)) / nwaves;
+    // contiguous run [tau0, tau1) of the walk index tau = group * 2^K + t for
+    // this wave, in units of (even, odd) step pairs, so that every resident
+    // wave gets the same work to within one pair and no second round runs at
+    // low occupancy; a run may start and end inside a group
+    const unsigned step = K >= 1 ? 2u : 1u;
+    const int64_t units = ((int64_t)1 << (H + K)) / step;
+    const int64_t tau0 = (units * wave / nwaves) * step;
+    const int64_t tau1 = (units * (wave + 1) / nwaves) * step;
 
     double tr = 0.0, ti = 0.0;  // lane total, lane sign applied at the end
-    if (g0 < g1) {
-        // ---- seed for group g0: delta_0 = +1, walk rows +1, lane row R-1-K-b is
-        // -1 iff bit b of the lane, group row R-1-K-B-h is -1 iff bit h of g0.
-        // Built from the +2A rows and halved (scaling by 2 is exact, so this
-        // equals seeding from A as BBFGPermanentCalculator.hpp:143-167).
+    if (tau0 < tau1) {
+        const int64_t g0 = tau0 >> K;
+        unsigned t0 = (unsigned)(tau0 & (int64_t)(W - 1));  // even when K >= 1
+        // ---- seed at (g0, t0): delta_0 = +1, walk row R-1-b is -1 iff bit b of
+        // the Gray code t0 ^ (t0 >> 1), lane row R-1-K-b is -1 iff bit b of the
+        // lane, group row R-1-K-B-h is -1 iff bit h of g0.  Built from the
+        // +/-2A rows and halved (scaling by 2 is exact, so this equals seeding
+        // from A as BBFGPermanentCalculator.hpp:143-167).
         double cr[NC], ci[NC];
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
@@ -158,8 +169,11 @@ void plain_sweep_kernel(PlainParams p) {
             cr[j] = v.x;
             ci[j] = v.y;
         }
-        for (int i = R - K; i < R; ++i)
-            add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + (size_t)i * rowb));
+        const unsigned gray0 = t0 ^ (t0 >> 1);
+        for (int i = R - K; i < R; ++i) {
+            const unsigned off = (((gray0 >> (R - 1 - i)) & 1u) ? neg : 0u) + (unsigned)i * rowb;
+            add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + off));
+        }
         for (int b = 0; b < B; ++b) {
             const int row = R - 1 - K - b;
             const double s = ((lane >> b) & 1) ? -1.0 : 1.0;
@@ -177,14 +191,17 @@ void plain_sweep_kernel(PlainParams p) {
         }
 
         for (int64_t g = g0;;) {
+            // this group's part of the run: t in [t0, t_stop)
+            const int64_t gbase = g << K;
+            const unsigned t_stop = (tau1 - gbase >= (int64_t)W) ? W : (unsigned)(tau1 - gbase);
             double pr = 0.0, pi = 0.0, mr = 0.0, mi = 0.0;
-            // t = 0: the group's first term (even Gray index)
+            // t = t0: even Gray index
             add_term<NC>(cr, ci, pr, pi);
             if (K >= 1) {
-                // t = 1: walk bit 0 (row R-1) goes to delta = -1
-                add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + low_m));
+                // t = t0 + 1: walk bit 0 (row R-1) goes to delta = -1 iff bit 1 of t0 is 0
+                add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + (((t0 >> 1) & 1u) ? low_p : low_m)));
                 add_term<NC>(cr, ci, mr, mi);
-                for (unsigned t = 2; t < W; t += 2) {
+                for (unsigned t = t0 + 2; t < t_stop; t += 2) {
                     const unsigned b = __builtin_ctz(t);
                     const unsigned nb = ((t >> b) ^ (t >> (b + 1))) & 1u;
                     const unsigned off = (nb ? neg : 0u) + (unsigned)(R - 1 - (int)b) * rowb;
@@ -199,10 +216,12 @@ void plain_sweep_kernel(PlainParams p) {
             const double gs = (__popcll((unsigned long long)g) & 1) ? -1.0 : 1.0;
             tr = fma(gs, pr - mr, tr);
             ti = fma(gs, pi - mi, ti);
-            if (++g >= g1) break;
-            // to the next group's seed: the walk ended at Gray code 10..0 (row
-            // R-K at delta = -1), restore it; then flip the group rows whose
-            // bits differ between g-1 and g (wave-uniform).
+            if (gbase + (int64_t)t_stop >= tau1) break;
+            ++g;
+            t0 = 0;
+            // to the next group's seed: the walk ended at t = 2^K - 1, Gray code
+            // 10..0 (row R-K at delta = -1), restore it; then flip the group rows
+            // whose bits differ between g-1 and g (wave-uniform).
             if (K >= 1) add_row<NC>(cr, ci, reinterpret_cast<const double2*>(Mb + (size_t)(R - K) * rowb));
             const int64_t diff = (g - 1) ^ g;
             for (int h = 0; h < H; ++h) {

@@ -139,17 +139,18 @@ def test_state_length_mismatch_raises():
 def test_plan_is_host_only():
     p = C.plain_plan(30, 30, waves_resident=3072)
     assert p["K"] + p["B"] + p["H"] == 29
-    # smallest power of two >= resident waves, two groups per wave
-    assert p["waves"] == 4096 and p["blocks"] * p["block"] == 64 * p["waves"]
-    assert 2 ** p["H"] == 2 * p["waves"] and p["K"] == 10
+    # exactly the resident waves, each a run of ~2.7 groups of 2^10 steps
+    assert p["waves"] == 3072 and p["blocks"] * p["block"] == 64 * p["waves"]
+    assert 2 ** p["H"] == 8192 and p["K"] == 10
     small = C.plain_plan(3, 3, waves_resident=3072)
-    assert small["K"] + small["B"] + small["H"] == 2 and small["waves"] == 1
+    assert small["K"] + small["B"] + small["H"] == 2 and small["waves"] == 2  # one step pair each
     for n in range(1, 41):
         for wr in (1, 7, 1024, 3072, 4096):
             q = C.plain_plan(n, n, waves_resident=wr)
             assert q["K"] + q["B"] + q["H"] == n - 1
             assert 0 <= q["K"] <= 14 and 0 <= q["B"] <= 6
-            assert 1 <= q["waves"] <= 2 * max(wr, 1) and q["waves"] <= 2 ** q["H"]
+            assert 1 <= q["waves"] <= max(wr, 1)
+            assert q["waves"] <= 2 ** (q["H"] + q["K"]) // (2 if q["K"] else 1)
             assert q["blocks"] * q["block"] == 64 * q["waves"]
 
 
