@@ -215,6 +215,30 @@ void build_table(const std::vector<int>& radix, std::vector<RepStep>& tab) {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// fn(lo, hi) over [0, n) in contiguous slices on up to 16 host threads, at
+// least `grain` items per thread; the caller's thread runs a slice itself when
+// no thread can be started.  fn must not throw.
+template <class F>
+static void host_parallel(int64_t n, int64_t grain, F&& fn) {
+    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+    const int64_t nthreads = std::min<int64_t>(hw, std::min<int64_t>(16, (n + grain - 1) / grain));
+    if (nthreads <= 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve((size_t)nthreads);
+    for (int64_t t = 0; t < nthreads; ++t) {
+        const int64_t lo = n * t / nthreads, hi = n * (t + 1) / nthreads;
+        try {
+            pool.emplace_back(fn, lo, hi);
+        } catch (const std::system_error&) {
+            fn(lo, hi);
+        }
+    }
+    for (std::thread& th : pool) th.join();
+}
+
 struct BatchBuffers {
     void* dbuf = nullptr;
     void* hbuf = nullptr;
@@ -469,8 +493,11 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     counts[0] = (unsigned)items.size();
     counts[1] = (unsigned)live.size();
     RepTask* T = reinterpret_cast<RepTask*>(h + B.off_tasks);
-    for (size_t k = 0; k < live.size(); ++k) {
-        const TaskPlan& P = plans[live[k]];
+    // descriptors and per-task data into the pinned staging buffer (disjoint
+    // regions per task; ~20 MB for 4096 tasks of up to 26 photons)
+    host_parallel((int64_t)live.size(), 256, [&](int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; ++k) {
+        const TaskPlan& P = plans[live[(size_t)k]];
         RepTask t;
         std::memset(&t, 0, sizeof(t));
         t.A = reinterpret_cast<const double2*>(d + B.off_data + a_off[k]);
@@ -491,9 +518,10 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
             t.hrad[hd] = P.hrad[(size_t)hd];
         }
         T[k] = t;
-        std::memcpy(h + B.off_data + a_off[k], P.A.data(), P.A.size() * sizeof(double));
-        std::memcpy(h + B.off_data + b_off[k], P.base.data(), P.base.size() * sizeof(double));
+        std::memcpy(h + B.off_data + a_off[(size_t)k], P.A.data(), P.A.size() * sizeof(double));
+        std::memcpy(h + B.off_data + b_off[(size_t)k], P.base.data(), P.base.size() * sizeof(double));
     }
+    });
     for (const auto& tb : tables)
         std::memcpy(h + B.off_data + tb.first, tb.second.data(), tb.second.size() * sizeof(RepStep));
     RepItem* I = reinterpret_cast<RepItem*>(h + B.off_items);
@@ -568,28 +596,51 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     const bool dd = precision == PQB_PREC_INF;
     std::vector<TaskPlan> plans((size_t)ntasks);
     std::vector<long double> steps((size_t)ntasks, 0.0L);
+    std::vector<int> photons((size_t)ntasks, 0);
+    // Validation and planning are per task and independent: large batches run
+    // them on host threads (4096 tasks are milliseconds of host work either
+    // way).  Errors come back per task; the first one by task index is
+    // reported, as a serial loop would.
+    std::vector<int> pst((size_t)ntasks, PQB_OK);
+    std::vector<std::string> perr((size_t)ntasks);
+    host_parallel(ntasks, 512, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            try {  // nothing may escape a worker thread
+                bool decided = false;
+                double v = 1.0;
+                const int st = check_task(tasks[i], algo, &decided, &v);
+                if (st) {
+                    pst[(size_t)i] = st;
+                    perr[(size_t)i] = t_err;  // t_err is per thread
+                    continue;
+                }
+                if (decided) {
+                    plans[(size_t)i].trivial = true;
+                    plans[(size_t)i].tre = v;
+                    continue;
+                }
+                std::vector<int64_t> rm(tasks[i].row_mult, tasks[i].row_mult + tasks[i].rows);
+                std::vector<int64_t> cm(tasks[i].col_mult, tasks[i].col_mult + tasks[i].cols);
+                rm.erase(std::remove(rm.begin(), rm.end(), 0), rm.end());
+                cm.erase(std::remove(cm.begin(), cm.end(), 0), cm.end());
+                steps[(size_t)i] = std::min(sweep_steps(rm), sweep_steps(cm));
+                photons[(size_t)i] = (int)std::accumulate(rm.begin(), rm.end(), (int64_t)0);
+            } catch (const std::exception& ex) {
+                pst[(size_t)i] = PQB_ERR_DEVICE;
+                perr[(size_t)i] = std::string("internal error: ") + ex.what();
+            }
+        }
+    });
+    for (int64_t i = 0; i < ntasks; ++i)
+        if (pst[(size_t)i]) {
+            t_err = (ntasks > 1 ? "task " + std::to_string(i) + ": " : std::string()) + perr[(size_t)i];
+            return pst[(size_t)i];
+        }
     long double total = 0.0L;
     int max_photons = 1;  // = the largest slot count of the batch
     for (int64_t i = 0; i < ntasks; ++i) {
-        bool decided = false;
-        double v = 1.0;
-        int st = check_task(tasks[i], algo, &decided, &v);
-        if (st) {
-            if (ntasks > 1) t_err = "task " + std::to_string(i) + ": " + t_err;
-            return st;
-        }
-        if (decided) {
-            plans[(size_t)i].trivial = true;
-            plans[(size_t)i].tre = v;
-            continue;
-        }
-        std::vector<int64_t> rm(tasks[i].row_mult, tasks[i].row_mult + tasks[i].rows);
-        std::vector<int64_t> cm(tasks[i].col_mult, tasks[i].col_mult + tasks[i].cols);
-        rm.erase(std::remove(rm.begin(), rm.end(), 0), rm.end());
-        cm.erase(std::remove(cm.begin(), cm.end(), 0), cm.end());
-        steps[(size_t)i] = std::min(sweep_steps(rm), sweep_steps(cm));
         total += steps[(size_t)i];
-        max_photons = std::max(max_photons, (int)std::accumulate(rm.begin(), rm.end(), (int64_t)0));
+        max_photons = std::max(max_photons, photons[(size_t)i]);
     }
     bool any_sweep = false;
     for (const TaskPlan& P : plans) any_sweep = any_sweep || !P.trivial;
@@ -607,12 +658,6 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     const int dd_range = rep_dd_range_for(max_photons);
     const int64_t lanes = dd ? (int64_t)batch_grid_locked(dd_range) * (kBlock / kWave) * rep_dd_walkers(dd_range)
                              : (int64_t)batch_grid_locked(2) * kBlock;
-    // Plans are independent: large batches are planned on several host
-    // threads (a plan is a few microseconds; 4096 of them dominate the host
-    // side of a launch otherwise).  Errors come back per task; the first one
-    // by task index is reported, as the serial loop would.
-    std::vector<int> pst((size_t)ntasks, PQB_OK);
-    std::vector<std::string> perr((size_t)ntasks);
     auto plan_range = [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; ++i) {
             if (plans[(size_t)i].trivial) continue;
@@ -633,24 +678,7 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
             }
         }
     };
-    const int64_t per_thread = 256;
-    const int64_t nthreads = std::min<int64_t>(
-        std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency()), std::min<int64_t>(16, (ntasks + per_thread - 1) / per_thread));
-    if (nthreads <= 1) {
-        plan_range(0, ntasks);
-    } else {
-        std::vector<std::thread> pool;
-        pool.reserve((size_t)nthreads);
-        for (int64_t t = 0; t < nthreads; ++t) {
-            const int64_t lo = ntasks * t / nthreads, hi = ntasks * (t + 1) / nthreads;
-            try {
-                pool.emplace_back(plan_range, lo, hi);
-            } catch (const std::system_error&) {
-                plan_range(lo, hi);  // no thread to be had: plan this slice here
-            }
-        }
-        for (std::thread& th : pool) th.join();
-    }
+    host_parallel(ntasks, 256, plan_range);
     for (int64_t i = 0; i < ntasks; ++i)
         if (pst[(size_t)i]) {
             t_err = (ntasks > 1 ? "task " + std::to_string(i) + ": " : std::string()) + perr[(size_t)i];

@@ -278,3 +278,17 @@ def test_batched_front_end_errors_without_device():
         b.calculate(lib=2)
     with pytest.raises(Exception, match="Wrong value set for permanent library"):
         b.calculate(lib=5)
+
+
+def test_batch_reports_the_first_failing_task():
+    """Validation of a large batch runs on host threads; the error reported is
+    still the one of the lowest task index (no device needed: the state sums
+    are checked before the engine starts)."""
+    A = np.eye(3, dtype=complex)
+    good = (A, np.array([1, 1, 1]), np.array([1, 1, 1]))
+    bad = (A, np.array([1, 1, 1]), np.array([1, 1, 2]))
+    tasks = [good] * 2000
+    for i in (1999, 1500, 700):
+        tasks[i] = bad
+    with pytest.raises(C.EngineError, match=r"^task 700: .*Number of input and output states should be equal"):
+        C.permanent_batch(tasks)
