@@ -286,14 +286,24 @@ static int ensure_batch_buffers_locked(size_t nt, size_t ni, size_t nd) {
     B.layout(std::max<size_t>(nt, std::max<size_t>(2 * B.cap_tasks, 64)),
              std::max<size_t>(ni, std::max<size_t>(2 * B.cap_items, 4096)),
              std::max<size_t>(nd, std::max<size_t>(2 * B.cap_data, (size_t)1 << 20)));
+    // on any failure below, leave no buffer behind: the capacity check above
+    // must not accept a half-built set on the next call
+    auto release = [&B]() {
+        if (B.dbuf) (void)hipFree(B.dbuf);
+        if (B.hbuf) (void)hipHostFree(B.hbuf);
+        B.dbuf = B.hbuf = nullptr;
+        B.cap_tasks = B.cap_items = B.cap_data = 0;
+    };
     hipError_t e = hipMalloc(&B.dbuf, B.total);
     if (e != hipSuccess) {
         B.dbuf = nullptr;
+        release();
         return hip_fail(e, "hipMalloc(batch)");
     }
     e = hipHostMalloc(&B.hbuf, B.off_data + B.cap_data, hipHostMallocDefault);
     if (e != hipSuccess) {
         B.hbuf = nullptr;
+        release();
         return hip_fail(e, "hipHostMalloc(batch)");
     }
     std::memset(B.hbuf, 0, B.off_data);
@@ -303,7 +313,10 @@ static int ensure_batch_buffers_locked(size_t nt, size_t ni, size_t nd) {
     // counter + header start at zero on the device
     e = hipMemcpyAsync(B.dbuf, B.hbuf, B.off_tasks, hipMemcpyHostToDevice, g_eng.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
-    if (e != hipSuccess) return hip_fail(e, "batch init");
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(e, "batch init");
+    }
     return PQB_OK;
 }
 
@@ -533,9 +546,12 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     if (e != hipSuccess) return hip_fail(e, "batch upload");
     hipGraphExec_t gx = nullptr;
     st = batch_graph_locked(range, &gx);
-    if (st) return st;
+    if (st) {
+        (void)hipStreamSynchronize(g_eng.stream);  // the upload may still read the staging buffer
+        return st;
+    }
     e = hipGraphLaunch(gx, g_eng.stream);
-    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    if (e != hipSuccess) return drain_fail(e, "hipGraphLaunch");
     std::vector<double> res(live.size() * 6);
     e = hipMemcpyAsync(res.data(), d + B.off_out, res.size() * sizeof(double), hipMemcpyDeviceToHost,
                        g_eng.stream);

@@ -40,6 +40,11 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(PQB_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+int drain_fail(hipError_t e, const char* what) {
+    if (g_eng.stream) (void)hipStreamSynchronize(g_eng.stream);
+    return hip_fail(e, what);
+}
+
 Engine g_eng;
 
 // ------------------------------------------------------------------ engine --
@@ -258,7 +263,7 @@ static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* resu
     p.counter = reinterpret_cast<unsigned*>(d);
     p.out = reinterpret_cast<unsigned long long*>(d + out_off);
     e = launch_plain_exact(C, p, (int)blocks, block, g_eng.stream);
-    if (e != hipSuccess) return hip_fail(e, "exact sweep launch");
+    if (e != hipSuccess) return drain_fail(e, "exact sweep launch");
     unsigned long long* hout = reinterpret_cast<unsigned long long*>(static_cast<char*>(g_eng.hbuf) + a_bytes);
     e = hipMemcpyAsync(hout, d + out_off, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g_eng.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
@@ -308,7 +313,7 @@ static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* o
     p.out = reinterpret_cast<double*>(d + out_off);
     p.scale = std::ldexp(1.0, -(R - 1));
     e = launch_plain(C, p, pl.blocks, pl.block, 1, g_eng.stream);
-    if (e != hipSuccess) return hip_fail(e, "plain sweep launch");
+    if (e != hipSuccess) return drain_fail(e, "plain sweep launch");
     double* hout = h + 2 * m2_elems;
     e = hipMemcpyAsync(hout, d + out_off, 6 * sizeof(double), hipMemcpyDeviceToHost, g_eng.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);

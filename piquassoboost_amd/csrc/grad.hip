@@ -205,7 +205,10 @@ static int grad_host_entry(const double* A, int64_t count, int64_t n, int64_t ld
     double2* dgrad = dperm + count;
     st = grad_launch(g, reinterpret_cast<const double2*>(d + a_off), nn, count, dperm, dgrad, d + part_off,
                      g_eng.stream);
-    if (st) return st;
+    if (st) {
+        (void)hipStreamSynchronize(g_eng.stream);  // the upload may still read the staging buffer
+        return st;
+    }
     // the staging buffer is reused for the results once the upload has been consumed
     e = hipMemcpyAsync(h, d + out_off, out_bytes, hipMemcpyDeviceToHost, g_eng.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
@@ -250,7 +253,10 @@ static int minors_host_entry(const double* A, int64_t rows, int64_t cols, int64_
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     st = grad_launch(g, reinterpret_cast<const double2*>(d + a_off), R, 1, nullptr,
                      reinterpret_cast<double2*>(d + out_off), d + part_off, g_eng.stream);
-    if (st) return st;
+    if (st) {
+        (void)hipStreamSynchronize(g_eng.stream);  // the upload may still read the staging buffer
+        return st;
+    }
     e = hipMemcpyAsync(h, d + out_off, out_bytes, hipMemcpyDeviceToHost, g_eng.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
     if (e != hipSuccess) return hip_fail(e, "column-minors sweep");

@@ -72,5 +72,9 @@ extern thread_local std::string t_err;
 
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+// hip_fail after waiting for the engine stream: for failures after an upload
+// from the staging buffer was queued, so that no copy still reads it when the
+// engine lock is released (engine lock held).
+int drain_fail(hipError_t e, const char* what);
 
 }  // namespace pqb
