@@ -96,6 +96,30 @@ def test_haar_n28_n30(haar_blocks, name):
     assert err < 1e-10
 
 
+@pytest.mark.parametrize("n", [29, 30])
+def test_full_size_invariants(haar_blocks, n):
+    """Size-independent properties at the full sizes (n=29 has its own launch
+    plan and no fixture): perm(A^T) = perm(A), perm(P A Q) = perm(A) for
+    permutations P, Q, perm(c A) = c^n perm(A) for a unit phase c, and
+    perm(conj A) = conj perm(A), each within 1e-10 relative.  Every variant
+    sums the same terms in another order or another walk."""
+    A = np.ascontiguousarray(haar_blocks["c2a_n30"][:n, :n])
+    base = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
+    rng = np.random.default_rng(3000 + n)
+    c = np.exp(0.7j)
+    cases = {
+        "transpose": (A.T, base),
+        "permuted": (A[rng.permutation(n)][:, rng.permutation(n)], base),
+        "phase": (c * A, c ** n * base),
+        "conjugate": (A.conj(), base.conjugate()),
+    }
+    for name, (M, want) in cases.items():
+        got = C.permanent(np.ascontiguousarray(M), C.ALGO_BBFG, C.PREC_FP64)
+        err = rel_err(got, want)
+        print(f"n={n} {name}: rel_err={err:.3e}", flush=True)
+        assert err < 1e-10, name
+
+
 def test_largest_size_n40_block_diagonal():
     """The largest order the kernels are built for (40 columns, 2^39 Gray steps): a
     block-diagonal of two n=20 Haar blocks with rows and columns permuted has

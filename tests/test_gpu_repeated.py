@@ -32,6 +32,14 @@ def test_30_photons_18_outputs_golden():
     # the same transition on the full 60-mode unitary with zero multiplicities
     full = BSU.BBFGRepeatedPermanentCalculatorDouble(d["U"], d["input_state"], d["output_state"]).calculate()
     assert rel_err(full, ref) < 1e-10
+    # and the expanded 30 x 30 matrix (rows repeated by their multiplicity)
+    # through the plain sweep: another kernel and another walk over the same sum
+    rows = np.repeat(np.arange(d["A"].shape[0]), d["row_mult"])
+    E = np.ascontiguousarray(d["A"][rows])
+    plain = C.permanent(E, C.ALGO_BBFG, C.PREC_FP64)  # = the repeated value (no multiplicity factor)
+    err = rel_err(plain, ref)
+    print(f"30 photons / 18 outputs, expanded 30 x 30 through the plain sweep: rel_err={err:.3e}", flush=True)
+    assert err < 1e-10
 
 
 SPECIAL = [
