@@ -269,11 +269,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHI <= 8 ? 
     const cptr<unsigned> order = to_const(p.order);
     const cptr<RepTask> tasks = to_const(p.tasks);
     const unsigned nitems = to_const(p.counts)[0];
-    for (;;) {
-        // every lane executes the atomic, only lane 0 adds (see rep_batch_kernel)
-        const unsigned got = atomicAdd(p.head, lane == 0 ? 1u : 0u);
-        const unsigned idx0 = __builtin_amdgcn_readlane(got, 0);
-        if (idx0 >= nitems) break;
+    // first item = wave index, then the counter offset by the wave count (see rep_batch_kernel)
+    const unsigned nwaves = gridDim.x * (blockDim.x / kWave);
+    unsigned idx0 = blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    while (idx0 < nitems) {
         const unsigned idx = order[idx0];
         const RepItem it = items[idx];
         const cptr<RepTask> T = tasks + it.task;
@@ -282,6 +281,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHI <= 8 ? 
         if (q != 0) acc = ddc{0.0, 0.0, 0.0, 0.0};
         const ddc v = wave_reduce(acc);
         if (lane == 0) p.partials[idx] = v;
+        // every lane executes the atomic, only lane 0 adds (see rep_batch_kernel)
+        const unsigned got = atomicAdd(p.head, lane == 0 ? 1u : 0u);
+        idx0 = nwaves + __builtin_amdgcn_readlane(got, 0);
     }
 }
 

@@ -197,23 +197,27 @@ void rep_batch_kernel(RepBatchParams p) {
     const cptr<unsigned> order = to_const(p.order);
     const cptr<RepTask> tasks = to_const(p.tasks);
     const unsigned nitems = to_const(p.counts)[0];
-    for (;;) {
-        // Every lane executes the atomic (uniform control flow); only lane 0
-        // adds 1, and lane 0's old value is the wave's item.  A lane-0-only
-        // branch around the atomic must not be used here: the structurized
-        // loop can then re-enter the item fetch for the other lanes without
-        // the atomic, and they spin on a stale index.
-        const unsigned got = atomicAdd(p.head, lane == 0 ? 1u : 0u);
-        const unsigned idx0 = __builtin_amdgcn_readlane(got, 0);  // wave-uniform
-        if (idx0 >= nitems) break;
-        unsigned idx = idx0;
-        idx = order[idx];  // largest items first; partials stay indexed by item
+    // A wave's first item is its wave index: no atomic, and the waves past the
+    // batch's item count leave at once (a small batch on the full persistent
+    // grid otherwise spent ~70 us on one contended counter).  Further items
+    // come from the counter, offset by the number of waves.
+    const unsigned nwaves = gridDim.x * (blockDim.x / kWave);
+    unsigned idx0 = blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    while (idx0 < nitems) {
+        const unsigned idx = order[idx0];  // largest items first; partials stay indexed by item
         const RepItem it = items[idx];
         const cptr<RepTask> T = tasks + it.task;
         double sr = 0.0, si = 0.0;
         rep_dispatch<LO, HI>(T->NS, T, p.binom, it.chunk0, it.count, lane, sr, si);
         const ddc v = wave_reduce(ddc_from(sr, si));
         if (lane == 0) p.partials[idx] = v;
+        // Every lane executes the atomic (uniform control flow); only lane 0
+        // adds 1, and lane 0's old value is the wave's next item.  A
+        // lane-0-only branch around the atomic must not be used here: the
+        // structurized loop can then re-enter the item fetch for the other
+        // lanes without the atomic, and they spin on a stale index.
+        const unsigned got = atomicAdd(p.head, lane == 0 ? 1u : 0u);
+        idx0 = nwaves + __builtin_amdgcn_readlane(got, 0);  // wave-uniform
     }
 }
 
