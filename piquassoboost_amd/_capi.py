@@ -197,8 +197,7 @@ def _task_array(tasks):
     fields: A, rows, cols, ld, row_mult, col_mult; pointers as addresses) and the
     arrays it points into (kept alive by the caller).  Inputs that are already
     C-contiguous complex128 / int64 are used in place."""
-    keep = []
-    recs = []
+    mats, rows, cols = [], [], []
     for i, (A, r, c) in enumerate(tasks):
         M = as_matrix(A)
         rv = as_state(r)
@@ -207,10 +206,24 @@ def _task_array(tasks):
             raise Exception(
                 f"task {i}: state lengths must match the matrix: len(output_state) == rows, len(input_state) == cols"
             )
-        keep.append((M, rv, cv))
-        recs.append((M.__array_interface__["data"][0], M.shape[0], M.shape[1], M.shape[1],
-                     rv.__array_interface__["data"][0], cv.__array_interface__["data"][0]))
-    return np.array(recs, dtype=np.int64).reshape(-1, 6), keep
+        mats.append(M)
+        rows.append(rv)
+        cols.append(cv)
+    n = len(mats)
+    nr = np.fromiter((v.shape[0] for v in rows), dtype=np.int64, count=n)
+    nc = np.fromiter((v.shape[0] for v in cols), dtype=np.int64, count=n)
+    # the occupation numbers of all tasks packed into two arrays: a pointer per
+    # task is then base + offset (an address lookup per array costs ~1 us)
+    rall = np.concatenate(rows) if n else np.zeros(0, np.int64)
+    call = np.concatenate(cols) if n else np.zeros(0, np.int64)
+    arr = np.empty((n, 6), dtype=np.int64)
+    arr[:, 0] = np.fromiter((M.__array_interface__["data"][0] for M in mats), dtype=np.int64, count=n)
+    arr[:, 1] = nr
+    arr[:, 2] = nc
+    arr[:, 3] = nc
+    arr[:, 4] = rall.__array_interface__["data"][0] + 8 * (np.cumsum(nr) - nr)
+    arr[:, 5] = call.__array_interface__["data"][0] + 8 * (np.cumsum(nc) - nc)
+    return arr, (mats, rall, call)
 
 
 def permanent_batch(tasks, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE, walk_length: int = 0) -> list:
