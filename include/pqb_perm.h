@@ -264,6 +264,14 @@ int pqb_perm_grad_device(const double* dA, int64_t count, int64_t n, double* d_p
  * out[0] = 1.  Host pointers, synchronous. */
 int pqb_perm_minors(const double* A, int64_t rows, int64_t cols, int64_t ld, double* out);
 
+/* The column minors of `count` rows x (rows+1) matrices in one launch: matrix b
+ * starts at A + 2 * b * stride doubles (stride >= rows * ld when count > 1) and
+ * out[2 * (b * cols + j)] is perm(A_b without column j).  The per-photon-step
+ * work of compute_pmf (samplingHelperFunctions.cpp:137-172) for many samples
+ * advanced together.  Host pointers, synchronous. */
+int pqb_perm_minors_batch(const double* A, int64_t count, int64_t rows, int64_t cols, int64_t ld,
+                          int64_t stride, double* out);
+
 /* Launch plan of the adjoint sweep: plan[0] = M (columns per lane), [1] = K
  * (register bits), [2] = L (lane bits), [3] = H (high bits), [4] = waves per
  * matrix, [5] = blocks, [6] = LDS bytes per block.  waves_resident > 0: plan

@@ -72,6 +72,8 @@ SIGNATURES = {
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "pqb_grad_plan": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p]),
     "pqb_perm_minors": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp]),
+    "pqb_perm_minors_batch": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                             ctypes.c_int64, _dp]),
     "pqb_perm_batch_device_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                        ctypes.POINTER(ctypes.c_size_t)]),
     "pqb_perm_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
@@ -396,4 +398,18 @@ def permanent_minors(A) -> np.ndarray:
     out = np.zeros(M.shape[1], dtype=np.complex128)
     _check(lib().pqb_perm_minors(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
                                  out.ctypes.data_as(_dp)))
+    return out
+
+
+def permanent_minors_batch(A) -> np.ndarray:
+    """For a (count, rows, rows+1) stack: out[b, j] = perm(A[b] without column j), one launch."""
+    M = np.ascontiguousarray(A, dtype=np.complex128)
+    if M.ndim != 3:
+        raise Exception("permanent_minors_batch: a (count, rows, rows + 1) array is needed")
+    count, rows, cols = M.shape
+    out = np.zeros((count, cols), dtype=np.complex128)
+    if count == 0:
+        return out
+    _check(lib().pqb_perm_minors_batch(M.ctypes.data_as(_dp), count, rows, cols, cols, rows * cols,
+                                       out.ctypes.data_as(_dp)))
     return out

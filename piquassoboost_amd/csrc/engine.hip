@@ -89,6 +89,7 @@ void Engine::free_buffers() {
 }
 
 int Engine::ensure_buffers_locked(size_t dneed, size_t hneed) {
+    const size_t old_d = dbytes, old_h = hbytes;
     if (dneed > dbytes) {
         if (dbuf) {
             (void)hipStreamSynchronize(stream);
@@ -96,7 +97,9 @@ int Engine::ensure_buffers_locked(size_t dneed, size_t hneed) {
             dbuf = nullptr;
             dbytes = 0;
         }
-        size_t sz = std::max<size_t>(dneed, (size_t)1 << 20);
+        // geometric growth: a caller whose sizes creep up (a sampler going photon by
+        // photon) reallocates a few times, not at every step
+        const size_t sz = std::max<size_t>(std::max<size_t>(dneed, 2 * old_d), (size_t)1 << 20);
         hipError_t e = hipMalloc(&dbuf, sz);
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)");
         // the grid-reduction counter at offset 0 must start at zero
@@ -112,7 +115,7 @@ int Engine::ensure_buffers_locked(size_t dneed, size_t hneed) {
             hbuf = nullptr;
             hbytes = 0;
         }
-        size_t sz = std::max<size_t>(hneed, (size_t)1 << 20);
+        const size_t sz = std::max<size_t>(std::max<size_t>(hneed, 2 * old_h), (size_t)1 << 20);
         hipError_t e = hipHostMalloc(&hbuf, sz, hipHostMallocDefault);
         if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)");
         hbytes = sz;

@@ -218,40 +218,49 @@ static int grad_host_entry(const double* A, int64_t count, int64_t n, int64_t ld
     return PQB_OK;
 }
 
-// Column minors of one rows x (rows+1) matrix: out[j] = perm(A without column j).
-static int minors_host_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, double* out) {
+// Column minors of `count` rows x (rows+1) matrices (matrix b at A + b * stride
+// elements): out[b * cols + j] = perm(A_b without column j).
+static int minors_host_entry(const double* A, int64_t count, int64_t rows, int64_t cols, int64_t ld,
+                             int64_t stride, double* out) {
     if (!out) return fail(PQB_ERR_ARG, "output pointer is NULL");
+    if (count < 1) return fail(PQB_ERR_ARG, "batch count must be positive");
     if (rows < 0 || cols != rows + 1) return fail(PQB_ERR_ARG, "column minors need a rows x (rows + 1) matrix");
     if (rows == 0) {  // the only minor is the empty matrix
-        out[0] = 1.0;
-        out[1] = 0.0;
+        for (int64_t b = 0; b < count; ++b) {
+            out[2 * b] = 1.0;
+            out[2 * b + 1] = 0.0;
+        }
         return PQB_OK;
     }
     if (cols > PQB_MAX_COLS)
         return fail(PQB_ERR_SIZE, "matrix has more than " + std::to_string(PQB_MAX_COLS) + " columns");
     if (!A) return fail(PQB_ERR_ARG, "matrix pointer is NULL");
     if (ld < cols) return fail(PQB_ERR_ARG, "leading dimension smaller than the number of columns");
+    if (count > 1 && stride < rows * ld) return fail(PQB_ERR_ARG, "matrix stride smaller than rows * ld");
     const int R = (int)rows, C = (int)cols;
 
     std::lock_guard<std::mutex> lk(g_eng.mu);
     GradPlan g;
-    int st = grad_plan_locked(R, 1, &g, true);
+    int st = grad_plan_locked(R, count, &g, true);
     if (st) return st;
-    const size_t a_bytes = (size_t)R * C * sizeof(double2);
-    const size_t out_bytes = (size_t)C * sizeof(double2);
-    const size_t part_bytes = grad_partials_bytes(R, 1, g.wpt, true);
+    const size_t mat = (size_t)R * C;
+    const size_t a_bytes = (size_t)count * mat * sizeof(double2);
+    const size_t out_bytes = (size_t)count * C * sizeof(double2);
+    const size_t part_bytes = grad_partials_bytes(R, count, g.wpt, true);
     const size_t a_off = kCounterBytes;  // offset 0 holds the plain sweep's grid counter
     const size_t out_off = a_off + ((a_bytes + 255) / 256) * 256;
     const size_t part_off = out_off + ((out_bytes + 255) / 256) * 256;
     st = g_eng.ensure_buffers_locked(part_off + part_bytes, std::max(a_bytes, out_bytes));
     if (st) return st;
     double* h = static_cast<double*>(g_eng.hbuf);
-    for (int i = 0; i < R; ++i)
-        std::memcpy(h + 2 * (size_t)i * C, A + 2 * (size_t)i * ld, (size_t)C * 2 * sizeof(double));
+    for (int64_t b = 0; b < count; ++b)
+        for (int i = 0; i < R; ++i)
+            std::memcpy(h + 2 * ((size_t)b * mat + (size_t)i * C), A + 2 * (b * stride + i * ld),
+                        (size_t)C * 2 * sizeof(double));
     char* d = static_cast<char*>(g_eng.dbuf);
     hipError_t e = hipMemcpyAsync(d + a_off, h, a_bytes, hipMemcpyHostToDevice, g_eng.stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
-    st = grad_launch(g, reinterpret_cast<const double2*>(d + a_off), R, 1, nullptr,
+    st = grad_launch(g, reinterpret_cast<const double2*>(d + a_off), R, count, nullptr,
                      reinterpret_cast<double2*>(d + out_off), d + part_off, g_eng.stream);
     if (st) {
         (void)hipStreamSynchronize(g_eng.stream);  // the upload may still read the staging buffer
@@ -339,7 +348,16 @@ int pqb_grad_plan(int64_t count, int64_t n, int64_t waves_resident, int64_t* pla
 
 int pqb_perm_minors(const double* A, int64_t rows, int64_t cols, int64_t ld, double* out) {
     try {
-        return minors_host_entry(A, rows, cols, ld, out);
+        return minors_host_entry(A, 1, rows, cols, ld, 0, out);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+int pqb_perm_minors_batch(const double* A, int64_t count, int64_t rows, int64_t cols, int64_t ld,
+                          int64_t stride, double* out) {
+    try {
+        return minors_host_entry(A, count, rows, cols, ld, stride, out);
     } catch (const std::exception& ex) {
         return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
     }

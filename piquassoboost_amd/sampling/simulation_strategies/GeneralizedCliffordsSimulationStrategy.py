@@ -61,15 +61,35 @@ def _addends_batched(U, sample, current_input, col_idx, perm_fn):
     return np.asarray(perm_fn(tasks), dtype=np.complex128) if tasks else np.zeros(0, np.complex128)
 
 
-def _addends_minors(U, sample, current_input, col_idx):
-    """All addends from one column-minors sweep: the permanent with one photon removed from
-    input mode idx is the minor of the expanded (k-1) x k matrix (rows: sample photons,
-    columns: input photons) without one column of mode idx."""
+def _expanded(U, sample, current_input):
+    """The expanded (k-1) x k matrix of a photon step (rows: sample photons, columns: input
+    photons) and the input mode of each column."""
     rows = np.repeat(np.arange(sample.shape[0]), sample)
     cols = np.repeat(np.arange(current_input.shape[0]), current_input)
-    minors = C.permanent_minors(np.ascontiguousarray(U[np.ix_(rows, cols)], dtype=np.complex128))
+    return np.ascontiguousarray(U[np.ix_(rows, cols)], dtype=np.complex128), cols
+
+
+def _addends_from_minors(minors, cols, col_idx):
+    """The permanent with one photon removed from input mode idx is the minor without one
+    (the first) column of mode idx."""
     first = {int(mode): j for j, mode in reversed(list(enumerate(cols)))}
     return np.array([minors[first[int(idx)]] for idx in col_idx], dtype=np.complex128)
+
+
+def _addends_minors(U, sample, current_input, col_idx):
+    """All addends from one column-minors sweep of the expanded matrix."""
+    E, cols = _expanded(U, sample, current_input)
+    return _addends_from_minors(C.permanent_minors(E), cols, col_idx)
+
+
+def _pmf_from_addends(U, sample, current_input, col_idx, addends):
+    """pmf[mdx] = |sum_idx current_input[idx] * addend[idx] * U[mdx, idx]|^2, normalised,
+    accumulated in the reference's order (samplingHelperFunctions.cpp:174-340)."""
+    perms = np.zeros(sample.shape[0], dtype=np.complex128)
+    for k, idx in enumerate(col_idx):
+        perms = perms + (current_input[idx] * addends[k]) * U[:, idx]
+    pmf = perms.real * perms.real + perms.imag * perms.imag
+    return pmf / pmf.sum()
 
 
 def compute_pmf(interferometer_matrix, sample, current_input, permanent_batch=None,
@@ -98,17 +118,16 @@ def compute_pmf(interferometer_matrix, sample, current_input, permanent_batch=No
         addends = _addends_batched(U, sample, current_input, col_idx, perm_fn)
     else:
         raise ValueError(f"unknown compute_pmf method {method!r}")
-    perms = np.zeros(sample.shape[0], dtype=np.complex128)
-    for k, idx in enumerate(col_idx):  # the reference's order of accumulation
-        perms = perms + (current_input[idx] * addends[k]) * U[:, idx]
-    pmf = perms.real * perms.real + perms.imag * perms.imag
-    return pmf / pmf.sum()
+    return _pmf_from_addends(U, sample, current_input, col_idx, addends)
 
 
 def sample_from_pmf(sample: np.ndarray, pmf: np.ndarray) -> None:
     """samplingHelperFunctions.cpp:350-367: first index whose running sum reaches the draw
     (index 0 when rounding leaves the total below it)."""
-    rand_num = rng_gen.random_double_0_1()
+    _sample_with_draw(sample, pmf, rng_gen.random_double_0_1())
+
+
+def _sample_with_draw(sample: np.ndarray, pmf: np.ndarray, rand_num: float) -> None:
     sampled_index = 0
     prob_sum = 0.0
     for idx in range(pmf.shape[0]):
@@ -162,12 +181,48 @@ class GeneralizedCliffordsBSimulationStrategy:
         """Tuple of `samples_number` output states (mode occupations, int64)."""
         input_state = np.asarray(input_state, dtype=np.int64)
         particles = modes_state_to_particle_state(input_state)
+        n_samples = max(0, int(samples_number))
+        if self._pmf_method == "minors" and self._perm is None and n_samples > 1:
+            return self._simulate_lockstep(input_state, particles, n_samples)
         samples = []
         for _ in range(max(0, int(samples_number))):
             sample = np.zeros(input_state.shape[0], dtype=np.int64)
             current_input = np.zeros(input_state.shape[0], dtype=np.int64)
             self._fill_r_sample(sample, current_input, particles.copy(), self._photons(input_state))
             samples.append(sample)
+        return tuple(samples)
+
+    def _simulate_lockstep(self, input_state, particles, n_samples: int):
+        """The samples advanced together: one column-minors launch per photon step covers
+        every sample still filling.  The random numbers depend on the seed alone (the
+        photon count, which particle enters at each step, the uniform draw of each step),
+        never on the permanents, so they are taken first in the order of the sequential
+        loop; the samples are those of the sequential loop."""
+        U = self.interferometer_matrix
+        m = input_state.shape[0]
+        inputs, draws = [], []  # per sample: input state after each step, uniform draw of each step
+        for _ in range(n_samples):
+            photons = self._photons(input_state)
+            working = particles.copy()
+            current = np.zeros(m, dtype=np.int64)
+            ins, ds = [], []
+            for _k in range(photons):
+                working = update_input_by_single_photon(current, working)
+                ins.append(current.copy())
+                ds.append(rng_gen.random_double_0_1())
+            inputs.append(ins)
+            draws.append(ds)
+        samples = [np.zeros(m, dtype=np.int64) for _ in range(n_samples)]
+        for k in range(max((len(d) for d in draws), default=0)):
+            active = [s for s in range(n_samples) if len(draws[s]) > k]
+            expanded = [_expanded(U, samples[s], inputs[s][k]) for s in active]
+            minors = C.permanent_minors_batch(np.stack([E for E, _ in expanded]))
+            for a, s in enumerate(active):
+                current = inputs[s][k]
+                col_idx = np.flatnonzero(current > 0)
+                addends = _addends_from_minors(minors[a], expanded[a][1], col_idx)
+                pmf = _pmf_from_addends(U, samples[s], current, col_idx, addends)
+                _sample_with_draw(samples[s], pmf, draws[s][k])
         return tuple(samples)
 
 

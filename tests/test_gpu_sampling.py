@@ -90,3 +90,36 @@ def test_column_minors_shapes():
     assert C.permanent_minors(np.zeros((0, 1))).tolist() == [1.0]
     with pytest.raises(C.EngineError):
         C.permanent_minors(np.ones((3, 3)))
+
+
+@pytest.mark.parametrize("R,count", [(0, 5), (1, 3), (4, 7), (8, 2), (13, 9), (19, 4), (24, 3)])
+def test_column_minors_batch(R, count):
+    """pqb_perm_minors_batch on a (count, R, R+1) stack equals pqb_perm_minors per matrix
+    (to 1e-12; the batch splits the sweep over other wave ranges) and, up to R = 13, the
+    oracle's minors."""
+    from conftest import random_complex
+    from piquassoboost_amd import _capi as C
+
+    rng = np.random.default_rng(700 + R)
+    stack = np.stack([random_complex(rng, R, R + 1) for _ in range(count)])
+    got = C.permanent_minors_batch(stack)
+    assert got.shape == (count, R + 1)
+    for b in range(count):
+        single = C.permanent_minors(stack[b])
+        scale = np.abs(single).max()
+        assert np.abs(got[b] - single).max() <= 1e-12 * scale, b
+        if 0 < R <= 13:
+            ref = np.array([oracle.bbfg(np.delete(stack[b], j, 1), "dd") for j in range(R + 1)])
+            assert np.abs(got[b] - ref).max() <= 1e-12 * np.abs(ref).max(), b
+    print(f"minors batch R={R} count={count}: ok", flush=True)
+
+
+def test_lockstep_sampler_equals_sequential_on_the_gpu():
+    """20 photons in 60 modes, 4 samples: the lock-step sampler (one batched minors launch
+    per photon step) and the per-sample loop (pmf_method="batch") give the same samples."""
+    U = haar_unitary(60, 1010)
+    inp = np.zeros(60, dtype=np.int64)
+    inp[:20] = 1
+    a = GeneralizedCliffordsBSimulationStrategy(U, 1010).simulate(inp, 4)
+    b = GeneralizedCliffordsBSimulationStrategy(U, 1010, pmf_method="batch").simulate(inp, 4)
+    assert [tuple(x) for x in a] == [tuple(x) for x in b]

@@ -106,3 +106,53 @@ def test_other_seed_gives_other_samples(seeding):
     s = GeneralizedCliffordsBSimulationStrategy(d["U"], 124, permanent_batch=oracle_batch)
     got = [tuple(int(x) for x in v) for v in s.simulate(d["input_state"], d["shots"])]
     assert Counter(got) != Counter(tuple(x) for x in d["lossless"])
+
+
+def oracle_minors_batch(Ms):
+    """Column minors of a (count, k-1, k) stack from the oracle (the stand-in for the GPU
+    entry the lock-step sampler calls)."""
+    Ms = np.asarray(Ms)
+    out = np.zeros((Ms.shape[0], Ms.shape[2]), dtype=complex)
+    for b, M in enumerate(Ms):
+        for j in range(M.shape[1]):
+            sub = np.delete(M, j, axis=1)
+            out[b, j] = 1.0 if sub.shape[0] == 0 else oracle.bbfg(np.ascontiguousarray(sub), "dd")
+    return out
+
+
+@pytest.fixture
+def oracle_minors(monkeypatch):
+    from piquassoboost_amd.sampling.simulation_strategies import GeneralizedCliffordsSimulationStrategy as G
+
+    monkeypatch.setattr(G.C, "permanent_minors_batch", oracle_minors_batch)
+
+
+def test_lockstep_sampler_reproduces_the_sequential_loop(seeding, oracle_minors):
+    """Several samples advanced together (one batched minors call per photon step) draw
+    the random numbers up front in the sequential loop's order: the samples equal those of
+    the sequential oracle-driven loop one by one, and the seeded fixture's multiset."""
+    d = seeding
+    # one generator is shared by all strategies (as pic::rng_gen): seed, then run, each in turn
+    lock = GeneralizedCliffordsBSimulationStrategy(d["U"], d["seed"])  # minors method: lock-step
+    a = [tuple(int(x) for x in v) for v in lock.simulate(d["input_state"], d["shots"])]
+    seq = GeneralizedCliffordsBSimulationStrategy(d["U"], d["seed"], permanent_batch=oracle_batch)
+    b = [tuple(int(x) for x in v) for v in seq.simulate(d["input_state"], d["shots"])]
+    assert a == b
+    assert Counter(a) == Counter(tuple(x) for x in d["lossless"])
+
+
+def test_lockstep_uniform_losses_reproduces_the_sequential_loop(seeding, oracle_minors):
+    """With losses the photon count differs per sample (drawn first); samples that are
+    complete leave the batch while the others continue."""
+    d = seeding
+    U = d["U"]
+    lossy = U @ np.diag([d["uniform_singular_value"]] * 5) @ U @ U.T
+    sv = np.linalg.svd(lossy)[1]
+    lock = GeneralizedCliffordsBUniformLossesSimulationStrategy(lossy / sv[0], sv[0], d["seed"])
+    a = [tuple(int(x) for x in v) for v in lock.simulate(d["input_state"], d["shots"])]
+    seq = GeneralizedCliffordsBUniformLossesSimulationStrategy(lossy / sv[0], sv[0], d["seed"],
+                                                               permanent_batch=oracle_batch)
+    b = [tuple(int(x) for x in v) for v in seq.simulate(d["input_state"], d["shots"])]
+    assert a == b
+    assert len(set(sum(x) for x in a)) > 1  # the photon counts do differ
+    assert Counter(a) == Counter(tuple(x) for x in d["uniform_losses"])
