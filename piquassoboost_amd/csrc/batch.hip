@@ -376,17 +376,24 @@ static int64_t g_walk[PQB_MAX_COLS + 1] = {0};
 // Target walk length for a task of S steps when the batch holds `total`
 // steps: long walks amortise the per-chunk seed, but there must be enough
 // chunks to keep every resident lane busy.
-static uint64_t choose_L(long double S, long double total, int64_t lanes) {
-    // A lane's time is (its chunks) x (walk + seed), the seed costing a few steps:
-    // with little work per lane one chunk each is fastest (a single 9e6-step task
-    // on the whole grid: 0.26 ms at 4 chunks per lane, 0.13 ms at about one);
-    // larger batches take 4 per lane so that uneven items balance.
+static uint64_t choose_L(long double S, long double total, int64_t lanes, bool fp64) {
+    // A lane's time is (its chunks) x (walk + per-chunk cost: the item fetch, the
+    // mixed-radix decomposition of the chunk index and the seed, tens of FP64 steps
+    // with many chunk digits).  With little work per lane one chunk each; in the FP64
+    // sweep chunks of up to 128 steps even if up to three quarters of the lanes stay
+    // idle (a single 9e6-step task on the whole grid: 0.26 ms at 4 chunks per lane,
+    // 0.137 ms at one chunk of ~45 steps per lane, 0.10 ms at 128-step chunks; a
+    // 4608-step task keeps its 16-step chunks).  The double-double step costs ~10x an
+    // FP64 step, so there the seed is cheap and short walks stay (n = 16: 0.046 ms
+    // against 0.19 ms with the floor).  Larger batches take 4 chunks per lane so that
+    // uneven items balance.
     const long double per_lane = std::max(total, S) / (long double)std::max<int64_t>(lanes, 1);
     const long double chunks_per_lane = per_lane < 1024.0L ? 1.0L : 4.0L;
     long double want_chunks =
         std::max<long double>(chunks_per_lane * (long double)lanes * S / std::max(total, S), 1.0L);
     long double L = S / want_chunks;
     L = std::min(L, S / 64.0L);  // at least one chunk per lane of a wave
+    if (fp64 && per_lane < 1024.0L) L = std::max(L, std::min(128.0L, 4.0L * S / (long double)std::max<int64_t>(lanes, 1)));
     L = std::min<long double>(std::max<long double>(L, 16.0L), 2048.0L);
     return (uint64_t)L;
 }
@@ -674,16 +681,21 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     const int dd_range = rep_dd_range_for(max_photons);
     const int64_t lanes = dd ? (int64_t)batch_grid_locked(dd_range) * (kBlock / kWave) * rep_dd_walkers(dd_range)
                              : (int64_t)batch_grid_locked(2) * kBlock;
+    const bool large_batch = total >= 1024.0L * (long double)lanes;  // steps per lane
     auto plan_range = [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; ++i) {
             if (plans[(size_t)i].trivial) continue;
             int64_t photons = 0;
             for (int64_t j = 0; j < tasks[i].cols; ++j) photons += tasks[i].col_mult[j];
-            // (the autotuned walk lengths are measured for the FP64 sweep)
-            const int64_t tuned = (!dd && photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons] : 0;
+            // The autotuned walk lengths are measured for the FP64 sweep on a large batch
+            // (thousands of steps per lane); a small batch, a single task above all, keeps
+            // the automatic rule (a 6e6-step task at the tuned 1024-step walk left all but
+            // ~6000 lanes idle: 0.32 ms against 0.10 ms).
+            const int64_t tuned =
+                (!dd && large_batch && photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons] : 0;
             const uint64_t L = L_override > 0 ? (uint64_t)L_override
                                : tuned > 0    ? (uint64_t)tuned
-                                              : choose_L(steps[(size_t)i], total, lanes);
+                                              : choose_L(steps[(size_t)i], total, lanes, !dd);
             try {  // nothing may escape a worker thread
                 pst[(size_t)i] = plan_task(tasks[i].A, tasks[i].rows, tasks[i].cols, tasks[i].ld,
                                            tasks[i].row_mult, tasks[i].col_mult, L, plans[(size_t)i]);
@@ -716,7 +728,7 @@ int run_plain_dd_locked(const double* A, int R, int C, int64_t ld, double* out) 
     const int range = rep_dd_range_for(C);  // the slots are the columns
     const int64_t walkers = (int64_t)batch_grid_locked(range) * (kBlock / kWave) * rep_dd_walkers(range);
     std::vector<TaskPlan> plans(1);
-    int st = plan_task(A, R, C, ld, rm.data(), cm.data(), choose_L(S, S, walkers), plans[0], true);
+    int st = plan_task(A, R, C, ld, rm.data(), cm.data(), choose_L(S, S, walkers, false), plans[0], true);
     if (st) return st;
     std::vector<double> out6;
     st = run_tasks_locked(plans, out6, true);
