@@ -306,6 +306,34 @@ def sampling_leg(photons: int = 20, modes: int = 60, shots: int = 8) -> dict:
     }
 
 
+def exact_leg(n: int = 26, count: int = 4, reps: int = 3) -> dict:
+    """Four n=26 Bernoulli(1/2) 0/1 matrices (seed 1003): the exact 128-bit integer sweep
+    (the double-precision codes route real integer matrices there), per call from the host
+    API, beside the FP64 sweep of the same matrices."""
+    from piquassoboost_amd import _capi as C
+
+    rng = np.random.default_rng(1003)
+    mats = [np.ascontiguousarray(rng.integers(0, 2, size=(n, n)).astype(np.float64)) for _ in range(count)]
+    for M in mats:
+        C.permanent_exact(M)
+    t0 = time.perf_counter()
+    values = []
+    for _ in range(reps):
+        values = [C.permanent_exact(M)[0] for M in mats]
+    exact_ms = (time.perf_counter() - t0) / (reps * count) * 1e3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for M in mats:
+            C.permanent(M, C.ALGO_BBFG, C.PREC_FP64)
+    fp64_ms = (time.perf_counter() - t0) / (reps * count) * 1e3
+    return {
+        "workload": f"{count} Bernoulli(1/2) 0/1 matrices {n}x{n} (seed 1003), exact integer permanents",
+        "exact_ms_per_call": exact_ms,
+        "fp64_sweep_ms_per_call": fp64_ms,
+        "permanents": [str(v) for v in values],
+    }
+
+
 def inf_leg(fp64_ms_n30: float, reps: int = 5) -> dict:
     """Double-double sweep (PQB_PREC_INF, the exact-arithmetic calculator codes):
     n=20 (40-mode Haar, seed 1001) and n=30 (the headline block) per permanent.
@@ -350,6 +378,7 @@ def main() -> int:
     ap.add_argument("--no-grad", action="store_true", help="skip the forward+backward leg")
     ap.add_argument("--no-sampling", action="store_true", help="skip the boson-sampling leg")
     ap.add_argument("--no-inf", action="store_true", help="skip the double-double (PQB_PREC_INF) leg")
+    ap.add_argument("--no-exact", action="store_true", help="skip the exact integer (0/1 matrices) leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
                          "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
@@ -452,6 +481,8 @@ def main() -> int:
         line["sampling"] = sampling_leg()
     if not a.no_inf:
         line["inf"] = inf_leg(ms_per_step)
+    if not a.no_exact:
+        line["exact"] = exact_leg()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A)
     print(json.dumps(line), flush=True)
