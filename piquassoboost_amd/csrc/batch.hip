@@ -61,10 +61,33 @@ static long double sweep_steps(const std::vector<int64_t>& v) {
     return s;
 }
 
+namespace {
+struct Digit {
+    int64_t src;  // oriented row
+    int m;        // multiplicity walked by this digit
+};
+// Temporaries of plan_task, kept per thread and reused: a batch plans thousands
+// of tasks on host threads, and fresh vectors per task made the allocator the
+// bottleneck.
+struct PlanScratch {
+    std::vector<int64_t> ri, rm, ci, cm, slot_col;
+    std::vector<Digit> digits, low, high, order_rows;
+    std::vector<size_t> ord;
+};
+}  // namespace
+
 int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int64_t* rmult,
               const int64_t* cmult, uint64_t target_L, TaskPlan& P, bool plain) {
     P = TaskPlan();
-    std::vector<int64_t> ri, rm, ci, cm;
+    thread_local PlanScratch sc;
+    std::vector<int64_t>& ri = sc.ri;
+    std::vector<int64_t>& rm = sc.rm;
+    std::vector<int64_t>& ci = sc.ci;
+    std::vector<int64_t>& cm = sc.cm;
+    ri.clear();
+    rm.clear();
+    ci.clear();
+    cm.clear();
     int64_t N = 0, Nc = 0;
     for (int64_t i = 0; i < rows; ++i)
         if (rmult[i] > 0) {
@@ -101,7 +124,8 @@ int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int
     const std::vector<int64_t>& col_m = transpose ? rm : cm;
 
     // column slots
-    std::vector<int64_t> slot_col;
+    std::vector<int64_t>& slot_col = sc.slot_col;
+    slot_col.clear();
     for (size_t c = 0; c < col_m.size(); ++c)
         for (int64_t k = 0; k < col_m[c]; ++k) slot_col.push_back((int64_t)c);
     const int NS = (int)slot_col.size();
@@ -110,19 +134,29 @@ int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int
     size_t anchor = 0;
     for (size_t r = 1; r < row_m.size(); ++r)
         if (row_m[r] < row_m[anchor]) anchor = r;
-    struct Digit { int64_t src; int m; };
-    std::vector<Digit> digits;
+    std::vector<Digit>& digits = sc.digits;
+    digits.clear();
     for (size_t r = 0; r < row_m.size(); ++r) {
         const int64_t m = (r == anchor) ? row_m[r] - 1 : row_m[r];
         if (m > 0) digits.push_back({(int64_t)r, (int)m});
     }
     // low / high split: largest radices first into the walk while it fits
-    std::vector<size_t> ord(digits.size());
+    std::vector<size_t>& ord = sc.ord;
+    ord.resize(digits.size());
     std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return digits[a].m > digits[b].m; });
+    // stable, largest first (insertion sort: a few dozen digits, no buffer to allocate)
+    for (size_t i = 1; i < ord.size(); ++i) {
+        const size_t v = ord[i];
+        size_t j = i;
+        for (; j > 0 && digits[ord[j - 1]].m < digits[v].m; --j) ord[j] = ord[j - 1];
+        ord[j] = v;
+    }
     uint64_t L = 1;
     for (;;) {
-        std::vector<Digit> low, high;
+        std::vector<Digit>& low = sc.low;
+        std::vector<Digit>& high = sc.high;
+        low.clear();
+        high.clear();
         L = 1;
         for (size_t k : ord) {
             const uint64_t rad = (uint64_t)digits[k].m + 1;
@@ -138,7 +172,8 @@ int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int
         if (nch < 4.0e9L) {
             P.nchunks = (uint64_t)nch;
             // rows of A: anchor, low digits in walk order, high digits
-            std::vector<Digit> order_rows;
+            std::vector<Digit>& order_rows = sc.order_rows;
+            order_rows.clear();
             order_rows.push_back({(int64_t)anchor, 1});
             for (const Digit& d : low) order_rows.push_back(d);
             for (const Digit& d : high) order_rows.push_back(d);
