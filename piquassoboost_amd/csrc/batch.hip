@@ -25,12 +25,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
 #include <system_error>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "pqb_perm.h"
@@ -216,13 +216,22 @@ int plan_task(const double* A, int64_t rows, int64_t cols, int64_t ld, const int
 }
 
 // Reflected mixed-radix Gray walk from k = 0 over digits with the given radices
-// (digit 0 moves fastest).  Entry t describes the move into index t.
-void build_table(const std::vector<int>& radix, std::vector<RepStep>& tab) {
+// (digit 0 moves fastest).  Entry t describes the move into index t; the table
+// has prod(radix) entries.  A step moves one digit by one, so the weight
+// prod C(m, k) changes by one exact ratio of binomials (an int64: the digits of a
+// walk hold at most 40 photons, prod C <= 2^40) and its sign alternates.
+static void build_table(const int* radix, size_t n, RepStep* tab) {
     uint64_t L = 1;
-    for (int r : radix) L *= (uint64_t)r;
-    tab.assign((size_t)L, RepStep{0u, 0u, 1.0});
-    const size_t n = radix.size();
-    std::vector<int> g(n, 0), o(n, 1);
+    for (size_t k = 0; k < n; ++k) L *= (uint64_t)radix[k];
+    int g[kMaxDigits], o[kMaxDigits];
+    int64_t bw[kMaxDigits][kMaxSlots + 1];
+    for (size_t k = 0; k < n; ++k) {
+        g[k] = 0;
+        o[k] = 1;
+        for (int v = 0; v < radix[k]; ++v) bw[k][v] = (int64_t)binom_u64(radix[k] - 1, v);
+    }
+    tab[0] = RepStep{0u, 0u, 1.0};
+    int64_t w = 1;
     for (uint64_t t = 1; t < L; ++t) {
         size_t d = 0;
         int nv = 0;
@@ -233,16 +242,11 @@ void build_table(const std::vector<int>& radix, std::vector<RepStep>& tab) {
             ++d;
         }
         const bool up = nv > g[d];
+        w = w / bw[d][g[d]] * bw[d][nv];
         g[d] = nv;
-        int64_t w = 1;
-        int sum = 0;
-        for (size_t k = 0; k < n; ++k) {
-            w *= (int64_t)binom_u64(radix[k] - 1, g[k]);
-            sum += g[k];
-        }
-        tab[(size_t)t].row = (uint32_t)(1 + d);
-        tab[(size_t)t].dir = up ? 1u : 0u;
-        tab[(size_t)t].w = (double)((sum & 1) ? -w : w);
+        tab[t].row = (uint32_t)(1 + d);
+        tab[t].dir = up ? 1u : 0u;
+        tab[t].w = (double)((t & 1) ? -w : w);
     }
 }
 
@@ -480,9 +484,13 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     }
 
     // ---- work items ----
-    struct ItemCost { uint64_t cost; unsigned idx; };
     std::vector<RepItem> items;
-    std::vector<ItemCost> costs;
+    // cost class of each item: items take few distinct costs (one per task shape
+    // and item size), ordered below by a counting pass instead of a sort
+    std::vector<unsigned> item_class;
+    std::vector<uint64_t> class_cost;
+    std::unordered_map<uint64_t, unsigned> class_of;
+    class_of.reserve(256);
     std::vector<std::pair<uint32_t, uint32_t>> task_items(live.size());
     // target Gray steps per lane (quad) per item: small items balance the tail of
     // an uneven batch (4096 tasks of 8..26 photons: replay 9.17 / 8.81 / 8.68 / 8.64 ms at 2048 / 1024 /
@@ -503,22 +511,56 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
         }
         if (n_default < target) cap = walkers * std::max<uint64_t>(1, (chunks + target * walkers - 1) / (target * walkers));
     }
+    {
+        size_t nitems = 0;
+        for (size_t k = 0; k < live.size(); ++k) {
+            const uint64_t per_item = std::min(per_item_of(plans[live[k]]), cap);
+            nitems += (size_t)((plans[live[k]].nchunks + per_item - 1) / per_item);
+        }
+        items.reserve(nitems);
+        item_class.reserve(nitems);
+    }
+    uint64_t last_cost = ~(uint64_t)0;
+    unsigned last_class = 0;
     for (size_t k = 0; k < live.size(); ++k) {
         const TaskPlan& P = plans[live[k]];
         const uint64_t per_item = std::min(per_item_of(P), cap);
         task_items[k].first = (uint32_t)items.size();
         for (uint64_t c0 = 0; c0 < P.nchunks; c0 += per_item) {
             const uint64_t cnt = std::min<uint64_t>(per_item, P.nchunks - c0);
-            costs.push_back({((cnt + walkers - 1) / walkers) * P.L * (uint64_t)(P.NS + 4), (unsigned)items.size()});
+            const uint64_t cost = ((cnt + walkers - 1) / walkers) * P.L * (uint64_t)(P.NS + 4);
+            if (cost != last_cost) {
+                const auto r = class_of.emplace(cost, (unsigned)class_cost.size());
+                if (r.second) class_cost.push_back(cost);
+                last_class = r.first->second;
+                last_cost = cost;
+            }
+            item_class.push_back(last_class);
             items.push_back(RepItem{(uint32_t)k, (uint32_t)cnt, (uint32_t)c0, 0u});
         }
         task_items[k].second = (uint32_t)items.size() - task_items[k].first;
     }
-    std::stable_sort(costs.begin(), costs.end(), [](const ItemCost& a, const ItemCost& b) { return a.cost > b.cost; });
+    // launch order: largest cost first, ties in item order
+    std::vector<unsigned> order(items.size());
+    {
+        std::vector<unsigned> rank(class_cost.size());
+        std::iota(rank.begin(), rank.end(), 0u);
+        std::sort(rank.begin(), rank.end(), [&](unsigned a, unsigned b) { return class_cost[a] > class_cost[b]; });
+        std::vector<unsigned> start(class_cost.size() + 1, 0u), pos_of(class_cost.size());
+        for (size_t r = 0; r < rank.size(); ++r) pos_of[rank[r]] = (unsigned)r;
+        for (unsigned c : item_class) ++start[pos_of[c] + 1];
+        for (size_t r = 0; r < rank.size(); ++r) start[r + 1] += start[r];
+        for (size_t i = 0; i < items.size(); ++i) order[start[pos_of[item_class[i]]]++] = (unsigned)i;
+    }
 
     // ---- data region: A, base per task, deduplicated step tables ----
-    std::map<std::vector<int>, size_t> table_off;
-    std::vector<std::pair<size_t, std::vector<RepStep>>> tables;  // (offset, table)
+    // step tables shared by the tasks whose walked radices agree, looked up by the
+    // radices as a short string (radices <= 41, up to 40 digits); built below,
+    // straight into the staging buffer
+    std::unordered_map<std::string, size_t> table_off;
+    table_off.reserve(64);
+    std::string key;
+    std::vector<std::pair<size_t, size_t>> tables;  // (offset, task whose radices it walks)
     std::vector<size_t> a_off(live.size()), b_off(live.size()), t_off(live.size());
     size_t nd = 0;
     for (size_t k = 0; k < live.size(); ++k) {
@@ -527,13 +569,13 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
         nd = align256(nd + P.A.size() * sizeof(double));
         b_off[k] = nd;
         nd = align256(nd + P.base.size() * sizeof(double));
-        auto it = table_off.find(P.low_radix);
+        key.assign(P.low_radix.begin(), P.low_radix.end());
+        auto it = table_off.find(key);
         if (it == table_off.end()) {
-            tables.emplace_back(nd, std::vector<RepStep>());
-            build_table(P.low_radix, tables.back().second);
-            table_off[P.low_radix] = nd;
+            tables.emplace_back(nd, k);
+            table_off.emplace(key, nd);
             t_off[k] = nd;
-            nd = align256(nd + tables.back().second.size() * sizeof(RepStep));
+            nd = align256(nd + (size_t)P.L * sizeof(RepStep));
         } else {
             t_off[k] = it->second;
         }
@@ -577,12 +619,16 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
         std::memcpy(h + B.off_data + b_off[(size_t)k], P.base.data(), P.base.size() * sizeof(double));
     }
     });
-    for (const auto& tb : tables)
-        std::memcpy(h + B.off_data + tb.first, tb.second.data(), tb.second.size() * sizeof(RepStep));
+    host_parallel((int64_t)tables.size(), 4, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            const std::vector<int>& rad = plans[live[tables[(size_t)i].second]].low_radix;
+            build_table(rad.data(), rad.size(), reinterpret_cast<RepStep*>(h + B.off_data + tables[(size_t)i].first));
+        }
+    });
     RepItem* I = reinterpret_cast<RepItem*>(h + B.off_items);
     std::memcpy(I, items.data(), items.size() * sizeof(RepItem));
     unsigned* O = reinterpret_cast<unsigned*>(h + B.off_order);
-    for (size_t k = 0; k < costs.size(); ++k) O[k] = costs[k].idx;
+    std::memcpy(O, order.data(), order.size() * sizeof(unsigned));
 
     hipError_t e = hipMemcpyAsync(d, h, B.off_data + nd, hipMemcpyHostToDevice, g_eng.stream);
     if (e != hipSuccess) return hip_fail(e, "batch upload");
