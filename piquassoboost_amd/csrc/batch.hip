@@ -24,7 +24,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -254,9 +256,69 @@ static void build_table(const int* radix, size_t n, RepStep* tab) {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Host worker threads kept for the life of the process (a 16-thread spawn and
+// join costs ~0.5 ms, and a batch call runs four parallel phases).  One parallel
+// phase at a time uses the pool; a phase that finds it busy (calls from several
+// host threads) starts threads of its own.  The pool object is never destroyed,
+// so no thread waits on a destroyed condition variable at process exit.
+struct HostPool {
+    std::mutex call_mu;  // one phase at a time
+    std::mutex mu;
+    std::condition_variable wake, fin;
+    const std::function<void(int64_t, int64_t)>* job = nullptr;
+    int64_t n = 0;
+    int slices = 0, next = 0, done = 0, workers = 0;
+
+    void worker() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            wake.wait(lk, [&] { return job != nullptr && next < slices; });
+            run_one(lk);
+        }
+    }
+    // claims the next slice (mu held, a slice left) and runs it with mu released
+    void run_one(std::unique_lock<std::mutex>& lk) {
+        const int s = next++;
+        const std::function<void(int64_t, int64_t)>* f = job;
+        const int64_t lo = n * s / slices, hi = n * (s + 1) / slices;
+        lk.unlock();
+        (*f)(lo, hi);
+        lk.lock();
+        if (++done == slices) fin.notify_all();
+    }
+    void run(int64_t count, int nslices, const std::function<void(int64_t, int64_t)>& f) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            while (workers < nslices - 1) {  // the calling thread runs slices too
+                try {
+                    std::thread(&HostPool::worker, this).detach();
+                } catch (const std::system_error&) {
+                    break;  // fewer workers: the caller claims what is left
+                }
+                ++workers;
+            }
+            job = &f;
+            n = count;
+            slices = nslices;
+            next = 0;
+            done = 0;
+        }
+        wake.notify_all();
+        std::unique_lock<std::mutex> lk(mu);
+        while (next < slices) run_one(lk);
+        fin.wait(lk, [&] { return done == slices; });
+        job = nullptr;
+    }
+};
+
+static HostPool& host_pool() {
+    static HostPool* pool = new HostPool;  // never destroyed (see above)
+    return *pool;
+}
+
 // fn(lo, hi) over [0, n) in contiguous slices on up to 16 host threads, at
-// least `grain` items per thread; the caller's thread runs a slice itself when
-// no thread can be started.  fn must not throw.
+// least `grain` items per slice; the caller's thread runs slices itself, and all
+// of them when no thread can be started.  fn must not throw.
 template <class F>
 static void host_parallel(int64_t n, int64_t grain, F&& fn) {
     const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
@@ -265,17 +327,24 @@ static void host_parallel(int64_t n, int64_t grain, F&& fn) {
         fn((int64_t)0, n);
         return;
     }
-    std::vector<std::thread> pool;
-    pool.reserve((size_t)nthreads);
+    HostPool& pool = host_pool();
+    std::unique_lock<std::mutex> busy(pool.call_mu, std::try_to_lock);
+    if (busy.owns_lock()) {
+        const std::function<void(int64_t, int64_t)> f = [&fn](int64_t lo, int64_t hi) { fn(lo, hi); };
+        pool.run(n, (int)nthreads, f);
+        return;
+    }
+    std::vector<std::thread> spawned;
+    spawned.reserve((size_t)nthreads);
     for (int64_t t = 0; t < nthreads; ++t) {
         const int64_t lo = n * t / nthreads, hi = n * (t + 1) / nthreads;
         try {
-            pool.emplace_back(fn, lo, hi);
+            spawned.emplace_back(fn, lo, hi);
         } catch (const std::system_error&) {
             fn(lo, hi);
         }
     }
-    for (std::thread& th : pool) th.join();
+    for (std::thread& th : spawned) th.join();
 }
 
 struct BatchBuffers {

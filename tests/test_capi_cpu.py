@@ -292,3 +292,32 @@ def test_batch_reports_the_first_failing_task():
         tasks[i] = bad
     with pytest.raises(C.EngineError, match=r"^task 700: .*Number of input and output states should be equal"):
         C.permanent_batch(tasks)
+
+
+def test_batch_validation_from_concurrent_host_threads():
+    """The host worker pool serves one parallel phase at a time; a phase that
+    finds it busy starts threads of its own.  Calls from 8 Python threads (ctypes
+    releases the GIL), 25 each, every one must report its own first bad task."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    A = np.eye(3, dtype=complex)
+    good = (A, np.array([1, 1, 1]), np.array([1, 1, 1]))
+    bad = (A, np.array([1, 1, 1]), np.array([1, 1, 2]))
+
+    def work(k):
+        first = 100 + 37 * k
+        tasks = [good] * 3000
+        for i in (first, first + 900, 2999):
+            tasks[i] = bad
+        for _ in range(25):
+            try:
+                C.permanent_batch(tasks)
+            except C.EngineError as e:
+                if not str(e).startswith(f"task {first}: "):
+                    return str(e)
+            else:
+                return "no error"
+        return None
+
+    with ThreadPoolExecutor(8) as ex:
+        assert list(ex.map(work, range(8))) == [None] * 8
