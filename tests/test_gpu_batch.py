@@ -99,3 +99,29 @@ def test_batched_front_end(uneven_batch4096):
         for k in range(0, 64, 9):
             A, rows, cols = tasks[k]
             assert rel_err(out[0, k], oracle.bbfg_repeated(A, cols, rows, "dd")) < 1e-10, (lib, k)
+
+
+def test_batches_from_concurrent_host_threads(uneven_batch4096):
+    """Batch calls from 4 Python threads at once (ctypes releases the GIL): their
+    host phases contend for the worker pool (a phase that finds it busy starts
+    its own threads) and their sweeps take the engine lock in turn.  Every call
+    returns the values of the same call made alone, bit for bit (the task list
+    and the state-array form of one half of the batch plan identically)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    U, pairs, tasks, res = uneven_batch4096
+    ins = np.stack([i for i, _ in pairs])
+    outs = np.stack([o for _, o in pairs])
+    want = [C.permanent_batch(tasks[h::2]) for h in range(2)]
+
+    def work(k):
+        h = k // 2
+        if k % 2:
+            got = C.permanent_batch_states(U, ins[h::2], outs[h::2]).tolist()
+        else:
+            got = C.permanent_batch(tasks[h::2])
+        return got == want[h]
+
+    with ThreadPoolExecutor(4) as ex:
+        for rep in range(3):
+            assert list(ex.map(work, range(4))) == [True] * 4, rep
