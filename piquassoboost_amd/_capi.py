@@ -193,6 +193,15 @@ def permanent_repeated(A, row_mult, col_mult, algo: int = ALGO_BBFG, precision: 
     return complex(out[0], out[1])
 
 
+def _address(M):
+    """Data address of a C-contiguous array: through the buffer protocol (~0.3 us),
+    else (read-only or empty arrays) through __array_interface__ (~1 us)."""
+    try:
+        return ctypes.addressof(ctypes.c_char.from_buffer(M))
+    except (TypeError, ValueError):
+        return M.__array_interface__["data"][0]
+
+
 def _task_array(tasks):
     """tasks: sequence of (matrix, row_mult, col_mult).  Returns an (n, 6) int64
     array laid out as n structs pqb_task of include/pqb_perm.h (six 8-byte
@@ -219,7 +228,7 @@ def _task_array(tasks):
     rall = np.concatenate(rows) if n else np.zeros(0, np.int64)
     call = np.concatenate(cols) if n else np.zeros(0, np.int64)
     arr = np.empty((n, 6), dtype=np.int64)
-    arr[:, 0] = np.fromiter((M.__array_interface__["data"][0] for M in mats), dtype=np.int64, count=n)
+    arr[:, 0] = np.fromiter((_address(M) for M in mats), dtype=np.int64, count=n)
     arr[:, 1] = nr
     arr[:, 2] = nc
     arr[:, 3] = nc
