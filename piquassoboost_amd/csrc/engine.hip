@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -256,11 +257,20 @@ static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* resu
     std::memcpy(g_eng.hbuf, ai.data(), a_bytes);
     hipError_t e = hipMemcpyAsync(d + a_off, g_eng.hbuf, a_bytes, hipMemcpyHostToDevice, g_eng.stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+    // every running column sum is bounded by its column's abs sum: 28^13, 1400^6 and
+    // (2^30)^2 are below 2^63 (integer_matrix admits column abs sums below 2^30)
+    int64_t cmax = 0;
+    for (int j = 0; j < C; ++j) {
+        int64_t s = 0;
+        for (int i = 0; i < R; ++i) s += std::llabs((long long)ai[(size_t)i * C + j]);
+        cmax = std::max(cmax, s);
+    }
     ExactParams p;
     p.A = reinterpret_cast<const int*>(d + a_off);
     p.R = R;
     p.K = K;
     p.H = H;
+    p.group = cmax <= 28 ? 13 : (cmax <= 1400 ? 6 : 2);
     p.nlanes = nlanes;
     p.partials = reinterpret_cast<unsigned long long*>(d + part_off);
     p.counter = reinterpret_cast<unsigned*>(d);

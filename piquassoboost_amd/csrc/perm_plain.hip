@@ -292,7 +292,18 @@ hipError_t launch_plain_prep(const double2* A, double2* M2, int n, int64_t count
 
 typedef unsigned __int128 u128;
 
-template <int NC>
+// a * q modulo 2^128 for a signed 64-bit q (two's complement wrap-around):
+// one 64 x 64 -> 128 product and one 64-bit low product of |q|, negated for q < 0
+__device__ __forceinline__ u128 mul_u128_s64(u128 a, int64_t q) {
+    const bool neg = q < 0;
+    const unsigned long long m = neg ? 0ull - (unsigned long long)q : (unsigned long long)q;
+    const u128 lo = (u128)(unsigned long long)a * m;
+    const unsigned long long hi = (unsigned long long)(a >> 64) * m + (unsigned long long)(lo >> 64);
+    const u128 r = ((u128)hi << 64) | (unsigned long long)lo;
+    return neg ? (u128)0 - r : r;
+}
+
+template <int NC, int G>
 __global__ __launch_bounds__(256) void plain_exact_kernel(ExactParams p) {
     __shared__ u128 red[256 / kWave];
 
@@ -328,9 +339,17 @@ __global__ __launch_bounds__(256) void plain_exact_kernel(ExactParams p) {
 #pragma unroll
             for (int j = 0; j < NC; ++j) c[j] += s * A[row * NC + j];
         }
-        u128 pr = (u128)(__int128)c[0];
+        // product of the column sums: G of them at a time exactly in int64 (the
+        // host bounds every |c_j| by the largest column abs sum, whose G-th power
+        // is below 2^63), each partial product folded into the 128-bit product
+        u128 pr = 0;
 #pragma unroll
-        for (int j = 1; j < NC; ++j) pr *= (u128)(__int128)c[j];
+        for (int j0 = 0; j0 < NC; j0 += G) {
+            int64_t q = c[j0];
+#pragma unroll
+            for (int j = j0 + 1; j < (j0 + G < NC ? j0 + G : NC); ++j) q *= (int64_t)c[j];
+            pr = (j0 == 0) ? (u128)(__int128)q : mul_u128_s64(pr, q);
+        }
         if (t & 1) acc -= pr; else acc += pr;
     }
     if (lane_neg) acc = (u128)0 - acc;
@@ -383,7 +402,12 @@ static hipError_t launch_plain_nc(const PlainParams& p, int nblocks, int bdim, i
 
 template <int NC>
 static hipError_t launch_exact_nc(const ExactParams& p, int nblocks, int bdim, hipStream_t st) {
-    hipLaunchKernelGGL(plain_exact_kernel<NC>, dim3(nblocks), dim3(bdim), 0, st, p);
+    switch (p.group) {
+        case 13: hipLaunchKernelGGL((plain_exact_kernel<NC, 13>), dim3(nblocks), dim3(bdim), 0, st, p); break;
+        case 6: hipLaunchKernelGGL((plain_exact_kernel<NC, 6>), dim3(nblocks), dim3(bdim), 0, st, p); break;
+        case 2: hipLaunchKernelGGL((plain_exact_kernel<NC, 2>), dim3(nblocks), dim3(bdim), 0, st, p); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -28,6 +28,8 @@ struct PlainParams {
 struct ExactParams {
     const int* A;        // R x NC integers, packed
     int R, K, H;
+    int group;           // column sums per 64-bit partial product (2, 6 or 13), with
+                         // (max_j sum_i |a_ij|)^group < 2^63
     int64_t nlanes;      // = 2^H
     unsigned long long* partials;  // two words per block
     unsigned* counter;
