@@ -34,8 +34,57 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 TRAFFIC_BYTES_N30 = 773028
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+
+
+def parity_vs_fixture(value: complex, A: np.ndarray, fixture: str, key: str, dd_key: str) -> dict:
+    """Relative error of `value` against the stored double-double value of the same
+    workload (the oracle's restated reference sweep), and whether the benchmark's
+    seeded matrix is bit-identical to the stored one."""
+    d = np.load(os.path.join(GOLDEN, fixture))
+    h = d[dd_key]
+    ref = complex(h[0] + h[1], h[2] + h[3])
+    return {"fixture": f"tests/golden/{fixture}:{dd_key}", "rel_err": abs(value - ref) / abs(ref),
+            "same_matrix": bool(np.array_equal(d[key], A))}
+
+
 def flops_plain(n: int) -> float:
     return (8 * n - 4) * 2.0 ** (n - 1)
+
+
+def host_cpus() -> dict:
+    """The host's CPUs as this process sees them: the model, the logical CPU count
+    (nproc), the CPUs the process may run on (affinity), the cgroup CPU quota and
+    the OMP_NUM_THREADS setting.  The baseline runs one thread per CPU the process
+    may use -- the reference sizes its TBB pool by hardware_concurrency()
+    (BBFGPermanentCalculator.hpp:120-122) -- capped by the cgroup quota and by
+    OMP_NUM_THREADS when either is set: a share of a larger machine gains nothing
+    from threads beyond its quota."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = affinity
+    if quota:
+        threads = min(threads, max(1, int(quota)))
+    if omp and omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity, "cgroup_quota": quota,
+            "omp_num_threads": omp, "threads": max(1, threads)}
 
 
 def cpu_baseline(A: np.ndarray, budget_s: float = 20.0) -> dict:
@@ -43,7 +92,8 @@ def cpu_baseline(A: np.ndarray, budget_s: float = 20.0) -> dict:
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    cpus = host_cpus()
+    threads = cpus["threads"]
     njobs = 4 * threads
     n = A.shape[0]
     # calibrate on a few jobs, then run a bounded number of whole jobs
@@ -65,6 +115,7 @@ def cpu_baseline(A: np.ndarray, budget_s: float = 20.0) -> dict:
         "kind": "port",
         "sample": f"{jobs} of {njobs} Gray-code chunks ({frac:.3f} of one n={n} permanent) in {dt:.2f} s, "
                   f"double, {threads} threads",
+        "host": cpus,
     }
 
 
@@ -143,7 +194,7 @@ def repeated_leg(reps: int = 10) -> dict:
         C.permanent_repeated(A, rows, cols)
     call_ms = (time.perf_counter() - t0) / reps * 1e3
     replay_ms = C.batch_replay_ms(reps) / reps
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cpus()["threads"]
     t0 = time.perf_counter()
     h = oracle.bbfg_repeated(A, cols, rows, "double", threads=threads)
     host_ms = (time.perf_counter() - t0) * 1e3
@@ -159,6 +210,7 @@ def repeated_leg(reps: int = 10) -> dict:
         "host_threads": threads,
         "gpu_vs_host": host_ms / call_ms,
         "perm": [v.real, v.imag],
+        "parity": parity_vs_fixture(v, A, "repeated_c3.npz", "A", "value_dd"),
         "host_rel_diff": abs(h - v) / abs(v),
     }
 
@@ -278,7 +330,7 @@ def sampling_leg(photons: int = 20, modes: int = 60, shots: int = 8) -> dict:
     U = haar_unitary(modes, 1010)
     inp = np.zeros(modes, np.int64)
     inp[:photons] = 1
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cpus()["threads"]
 
     def oracle_batch(tasks):
         return [1.0 + 0j if A.size == 0 else oracle.bbfg_repeated(A, c, r, "double", threads=threads)
@@ -355,7 +407,7 @@ def inf_leg(fp64_ms_n30: float, reps: int = 5) -> dict:
         out[f"n{n}_perm"] = [v.real, v.imag]
     out["n30_vs_fp64_sweep"] = out["n30_ms"] / fp64_ms_n30
     A = np.ascontiguousarray(haar_unitary(40, 1001)[:20, :20])
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cpus()["threads"]
     njobs = 64 * threads
     jobs = 4 * threads
     t0 = time.perf_counter()
@@ -379,6 +431,9 @@ def main() -> int:
     ap.add_argument("--no-sampling", action="store_true", help="skip the boson-sampling leg")
     ap.add_argument("--no-inf", action="store_true", help="skip the double-double (PQB_PREC_INF) leg")
     ap.add_argument("--no-exact", action="store_true", help="skip the exact integer (0/1 matrices) leg")
+    ap.add_argument("--fma-probe", action="store_true",
+                    help="instead of the benchmark, run the FP64 FMA issue-rate probe at 1..4 waves per SIMD "
+                         "(the FP64 VALU ceiling of the device; profile it for the clock)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
                          "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
@@ -392,6 +447,18 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 10}), flush=True)
+        return 0
+    if a.fma_probe:
+        import torch  # noqa: F401  (one HIP runtime: torch's)
+
+        from piquassoboost_amd import _capi as C
+
+        res = {}
+        for w in (1, 2, 3, 4):
+            ms, tf = C.probe_fp64_fma(w)
+            res[f"waves_per_simd_{w}"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP64_PEAK_TFLOPS}
+            print(f"fp64 fma probe, {w} waves/SIMD: {ms:.2f} ms, {tf:.1f} TFLOP/s", file=sys.stderr, flush=True)
+        print(json.dumps({"fp64_fma_probe": res, "peak": FP64_PEAK_TFLOPS}), flush=True)
         return 0
     if a.gpus != 1:
         print("bench.py measures one GPU: the permanent engine runs on a single MI355X in this build "
@@ -470,6 +537,7 @@ def main() -> int:
         },
         "event_ms_per_perm": ms_per_step,
         "perm": [perm.real, perm.imag],
+        "parity": parity_vs_fixture(perm, A, "haar_blocks.npz", "c2a_n30", "c2a_n30_dd"),
     }
     if not a.no_repeated:
         line["repeated"] = repeated_leg()

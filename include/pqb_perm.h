@@ -108,6 +108,22 @@ int pqb_init(int device);
 int pqb_perm(const double* A, int64_t rows, int64_t cols, int64_t ld, int algo,
              int precision, double* out);
 
+/* Plain permanents of a list of `count` host matrices of any shapes: matrix i
+ * at A[i], rows[i] x cols[i], row stride ld[i]; out[2i], out[2i+1] = its
+ * permanent.  Replaces the list surface of the plain Python calculator
+ * (GlynnPermanentCalculator_Wrapper.hpp:204-293: a list of matrices in, a list
+ * of permanents out, one calculator call per matrix).  Same algo and precision
+ * codes, edge rules and routes as pqb_perm; the work is regrouped: matrices
+ * of one shape on the FP64 route share one launch (a grid row per matrix, each
+ * with the launch plan of a single call, so each value is bit-identical to
+ * the pqb_perm call on that matrix), the double-double route (PQB_PREC_INF)
+ * takes one batched launch (the correctly rounded sum, as per call, whenever
+ * the sum cancels by less than ~2^45), exact-route integer matrices one sweep
+ * each.  A failing matrix fails the call; the message names its index.
+ * Synchronous. */
+int pqb_perm_list(const double* const* A, const int64_t* rows, const int64_t* cols, const int64_t* ld,
+                  int64_t count, int algo, int precision, double* out);
+
 /* Exact integer permanent of an integer-valued matrix (PQB_PREC_EXACT rules).
  * words[0], words[1] = low and high 64-bit words of the two's complement
  * 128-bit permanent; out = its correctly rounded double value.  No reference
@@ -119,10 +135,11 @@ int pqb_perm_exact(const double* A, int64_t rows, int64_t cols, int64_t ld,
 /* Host-only eligibility check of the exact integer route (no device needed):
  * PQB_OK with log2_bound = log2 of a proven upper bound on
  * 2^(rows-1) |perm(A)| when the matrix is real and integer valued and the
- * bound is below 2^126.5; PQB_ERR_EXACT otherwise.  Bounds used:
- * |perm(A)| <= min(prod of row abs sums, prod of column abs sums), and
- * Bregman's bound prod_i (r_i!)^(1/r_i) for square matrices with entries in
- * {-1, 0, 1}.  No reference counterpart. */
+ * bound is below 2^126.5; PQB_ERR_EXACT otherwise.  Bounds used: for every
+ * shape, the sweep's sum is at most 2^(rows-1) prod_j (column j abs sum); for
+ * square matrices also |perm(A)| <= prod of row abs sums, and Bregman's bound
+ * prod_i (r_i!)^(1/r_i) for entries in {-1, 0, 1}.  No reference
+ * counterpart. */
 int pqb_exact_bound(const double* A, int64_t rows, int64_t cols, int64_t ld, double* log2_bound);
 
 /* ---- repeated-row permanent -----------------------------------------------
@@ -284,6 +301,13 @@ int pqb_grad_plan(int64_t count, int64_t n, int64_t waves_resident, int64_t* pla
  * before the first and after the last launch.  dA, d_out as above. */
 int pqb_time_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
                          int iters, double* ms);
+
+/* FP64 issue-rate probe (measurement only): `waves_per_simd` waves on every
+ * SIMD each run `iters` x 8 independent v_fma_f64 chains; reports the timed
+ * launch's milliseconds (HIP events on the engine stream) and its FP64
+ * TFLOP/s (2 flops per FMA).  The ceiling any FP64 VALU kernel can reach on
+ * this device at that occupancy.  No reference counterpart. */
+int pqb_probe_fp64_fma(int waves_per_simd, int64_t iters, double* ms, double* tflops);
 
 #ifdef __cplusplus
 }

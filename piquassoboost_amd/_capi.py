@@ -45,6 +45,8 @@ SIGNATURES = {
     "pqb_init": (ctypes.c_int, [ctypes.c_int]),
     "pqb_perm": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
                                 ctypes.c_int, _dp]),
+    "pqb_perm_list": (ctypes.c_int, [ctypes.c_void_p, _i64p, _i64p, _i64p, ctypes.c_int64, ctypes.c_int,
+                                     ctypes.c_int, _dp]),
     "pqb_perm_exact": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _dp]),
     "pqb_exact_bound": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp]),
     "pqb_perm_repeated": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p,
@@ -76,6 +78,7 @@ SIGNATURES = {
                                              ctypes.c_int64, _dp]),
     "pqb_perm_batch_device_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                        ctypes.POINTER(ctypes.c_size_t)]),
+    "pqb_probe_fp64_fma": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _dp, _dp]),
     "pqb_perm_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
 }
@@ -151,6 +154,35 @@ def as_matrix(A) -> np.ndarray:
     return M
 
 
+def as_rows(A) -> tuple[np.ndarray, int]:
+    """(matrix, ld): a complex128 2-D array whose rows are contiguous and evenly
+    spaced (a C-contiguous array, or a block / row slice of one) is passed in
+    place with its row stride as ld, as pic::matrix carries `stride`
+    (matrix_base.hpp:48-49); anything else is copied to a C-contiguous array
+    (ld = cols)."""
+    if isinstance(A, np.ndarray) and A.dtype == np.complex128 and A.ndim == 2 and A.size > 0:
+        s0, s1 = A.strides
+        if s1 == 16 and s0 > 0 and s0 % 16 == 0 and s0 // 16 >= A.shape[1] and A.flags.aligned:
+            return A, s0 // 16
+    M = as_matrix(A)
+    return M, M.shape[1]
+
+
+def _stack_strides(B) -> tuple[np.ndarray, int, int]:
+    """(stack, ld, stride) of a (count, rows, cols) complex128 stack in elements,
+    kept in place when rows are contiguous and evenly spaced, else copied."""
+    if isinstance(B, np.ndarray) and B.dtype == np.complex128 and B.ndim == 3 and B.size > 0:
+        s0, s1, s2 = B.strides
+        count, rows, cols = B.shape
+        if (s2 == 16 and s1 > 0 and s1 % 16 == 0 and s1 // 16 >= cols and s0 % 16 == 0
+                and (count == 1 or s0 // 16 >= rows * (s1 // 16)) and B.flags.aligned):
+            return B, s1 // 16, (s0 // 16 if count > 1 else rows * (s1 // 16))
+    C = np.ascontiguousarray(B, dtype=np.complex128)
+    if C.ndim != 3:
+        return C, 0, 0
+    return C, C.shape[2], C.shape[1] * C.shape[2]
+
+
 def as_state(s) -> np.ndarray:
     v = np.ascontiguousarray(s, dtype=np.int64)
     if v.ndim != 1:
@@ -163,24 +195,45 @@ def device_count() -> int:
 
 
 def permanent(A, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE) -> complex:
-    M = as_matrix(A)
+    M, ld = as_rows(A)
     out = (ctypes.c_double * 2)()
-    _check(lib().pqb_perm(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1], algo, precision, out))
+    _check(lib().pqb_perm(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], ld, algo, precision, out))
     return complex(out[0], out[1])
+
+
+def permanent_list(mats, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE) -> list:
+    """Plain permanents of a list of matrices (any shapes) in one call: each value
+    equals permanent(m, algo, precision) bit for bit; matrices of one shape share a
+    launch (pqb_perm_list)."""
+    mats = list(mats)
+    n = len(mats)
+    if n == 0:
+        return []
+    keep = [as_rows(m) for m in mats]
+    ptrs = np.fromiter((_address(M) if M.size else 0 for M, _ in keep), dtype=np.int64, count=n)
+    rows = np.fromiter((M.shape[0] for M, _ in keep), dtype=np.int64, count=n)
+    cols = np.fromiter((M.shape[1] for M, _ in keep), dtype=np.int64, count=n)
+    lds = np.fromiter((ld for _, ld in keep), dtype=np.int64, count=n)
+    out = np.zeros(2 * n, dtype=np.float64)
+    _check(lib().pqb_perm_list(ctypes.c_void_p(ptrs.ctypes.data), rows.ctypes.data_as(_i64p),
+                               cols.ctypes.data_as(_i64p), lds.ctypes.data_as(_i64p), n, algo, precision,
+                               out.ctypes.data_as(_dp)))
+    del keep
+    return (out[0::2] + 1j * out[1::2]).tolist()
 
 
 def permanent_exact(A) -> tuple[int, float]:
     """Exact integer permanent (two's complement 128-bit) and its rounded double."""
-    M = as_matrix(A)
+    M, ld = as_rows(A)
     w = (ctypes.c_int64 * 2)()
     out = (ctypes.c_double * 2)()
-    _check(lib().pqb_perm_exact(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1], w, out))
+    _check(lib().pqb_perm_exact(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], ld, w, out))
     v = (int(w[1]) << 64) | (int(w[0]) & 0xFFFFFFFFFFFFFFFF)
     return v, out[0]
 
 
 def permanent_repeated(A, row_mult, col_mult, algo: int = ALGO_BBFG, precision: int = PREC_DOUBLE) -> complex:
-    M = as_matrix(A)
+    M, ld = as_rows(A)
     r = as_state(row_mult)
     c = as_state(col_mult)
     if r.shape[0] != M.shape[0] or c.shape[0] != M.shape[1]:
@@ -188,7 +241,7 @@ def permanent_repeated(A, row_mult, col_mult, algo: int = ALGO_BBFG, precision: 
             "state lengths must match the matrix: len(output_state) == rows, len(input_state) == cols"
         )
     out = (ctypes.c_double * 2)()
-    _check(lib().pqb_perm_repeated(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
+    _check(lib().pqb_perm_repeated(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], ld,
                                    r.ctypes.data_as(_i64p), c.ctypes.data_as(_i64p), algo, precision, out))
     return complex(out[0], out[1])
 
@@ -206,11 +259,12 @@ def _task_array(tasks):
     """tasks: sequence of (matrix, row_mult, col_mult).  Returns an (n, 6) int64
     array laid out as n structs pqb_task of include/pqb_perm.h (six 8-byte
     fields: A, rows, cols, ld, row_mult, col_mult; pointers as addresses) and the
-    arrays it points into (kept alive by the caller).  Inputs that are already
-    C-contiguous complex128 / int64 are used in place."""
-    mats, rows, cols = [], [], []
+    arrays it points into (kept alive by the caller).  Matrices whose rows are
+    contiguous complex128 (see as_rows) and C-contiguous int64 states are used
+    in place."""
+    mats, lds, rows, cols = [], [], [], []
     for i, (A, r, c) in enumerate(tasks):
-        M = as_matrix(A)
+        M, ld = as_rows(A)
         rv = as_state(r)
         cv = as_state(c)
         if rv.shape[0] != M.shape[0] or cv.shape[0] != M.shape[1]:
@@ -218,6 +272,7 @@ def _task_array(tasks):
                 f"task {i}: state lengths must match the matrix: len(output_state) == rows, len(input_state) == cols"
             )
         mats.append(M)
+        lds.append(ld)
         rows.append(rv)
         cols.append(cv)
     n = len(mats)
@@ -231,7 +286,7 @@ def _task_array(tasks):
     arr[:, 0] = np.fromiter((_address(M) for M in mats), dtype=np.int64, count=n)
     arr[:, 1] = nr
     arr[:, 2] = nc
-    arr[:, 3] = nc
+    arr[:, 3] = np.fromiter(lds, dtype=np.int64, count=n)
     arr[:, 4] = rall.__array_interface__["data"][0] + 8 * (np.cumsum(nr) - nr)
     arr[:, 5] = call.__array_interface__["data"][0] + 8 * (np.cumsum(nc) - nc)
     return arr, (mats, rall, call)
@@ -258,7 +313,7 @@ def permanent_batch_states(U, inputs, outputs, algo: int = ALGO_BBFG, precision:
     launch: inputs is (count, cols), outputs (count, rows); pair i is the repeated-row
     permanent with row multiplicities outputs[i] and column multiplicities inputs[i]
     (BatchednPermanentCalculator::calculate without per-pair matrices)."""
-    M = as_matrix(U)
+    M, ld = as_rows(U)
     ins = np.ascontiguousarray(inputs, dtype=np.int64)
     outs = np.ascontiguousarray(outputs, dtype=np.int64)
     if ins.ndim == 1:
@@ -271,7 +326,7 @@ def permanent_batch_states(U, inputs, outputs, algo: int = ALGO_BBFG, precision:
         raise Exception("state lengths must match the interferometer: outputs per row, inputs per column")
     count = ins.shape[0]
     out = np.zeros(2 * count, dtype=np.float64)
-    _check(lib().pqb_perm_batch_states(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
+    _check(lib().pqb_perm_batch_states(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], ld,
                                        ins.ctypes.data_as(_i64p), outs.ctypes.data_as(_i64p), count, algo,
                                        precision, out.ctypes.data_as(_dp)))
     return out[0::2] + 1j * out[1::2]
@@ -298,20 +353,20 @@ def batch_replay_ms(iters: int) -> float:
 
 def repeated_plan(A, row_mult, col_mult, walk_length: int) -> dict:
     """Host-only decomposition of one repeated task (no device needed)."""
-    M = as_matrix(A)
+    M, ld = as_rows(A)
     r = as_state(row_mult)
     c = as_state(col_mult)
     info = (ctypes.c_int64 * 6)()
-    _check(lib().pqb_repeated_plan(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
+    _check(lib().pqb_repeated_plan(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], ld,
                                    r.ctypes.data_as(_i64p), c.ctypes.data_as(_i64p), walk_length, info))
     return dict(zip(["slots", "rows", "low", "high", "walk", "chunks"], [int(v) for v in info]))
 
 
 def exact_bound(A) -> tuple[bool, float]:
     """(eligible, log2 bound) of the exact integer route; host only."""
-    M = as_matrix(A)
+    M, ld = as_rows(A)
     b = ctypes.c_double(0.0)
-    rc = lib().pqb_exact_bound(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1], ctypes.byref(b))
+    rc = lib().pqb_exact_bound(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], ld, ctypes.byref(b))
     if rc == PQB_ERR_EXACT and b.value == 0.0 and "integer-valued" in lib().pqb_last_error().decode():
         return False, float("nan")
     if rc not in (PQB_OK, PQB_ERR_EXACT):
@@ -345,25 +400,25 @@ def time_permanent_device(dA_ptr: int, rows: int, cols: int, d_out_ptr: int, ite
 
 def permanent_grad(A) -> tuple[complex, np.ndarray]:
     """(perm(A), G) with G[i, j] = d perm / d a_ij = perm of the (i, j) minor; A square."""
-    M = as_matrix(A)
+    M, ld = as_rows(A)
     n = M.shape[0]
     if M.shape[1] != n:
         raise EngineError(PQB_ERR_ARG, "the gradient needs a square matrix")
     out = (ctypes.c_double * 2)()
     G = np.zeros((n, n), dtype=np.complex128)
-    _check(lib().pqb_perm_grad(M.ctypes.data_as(_dp), n, n, out, G.ctypes.data_as(_dp)))
+    _check(lib().pqb_perm_grad(M.ctypes.data_as(_dp), n, ld, out, G.ctypes.data_as(_dp)))
     return complex(out[0], out[1]), G
 
 
 def permanent_grad_batch(As) -> tuple[np.ndarray, np.ndarray]:
     """Batch of equal-size square matrices (count x n x n) -> (perms[count], grads[count, n, n])."""
-    B = np.ascontiguousarray(As, dtype=np.complex128)
+    B, ld, stride = _stack_strides(As)
     if B.ndim != 3 or B.shape[1] != B.shape[2]:
         raise EngineError(PQB_ERR_ARG, "expected a count x n x n batch of square matrices")
     count, n = B.shape[0], B.shape[1]
     perms = np.zeros(count, dtype=np.complex128)
     G = np.zeros((count, n, n), dtype=np.complex128)
-    _check(lib().pqb_perm_grad_batch(B.ctypes.data_as(_dp), count, n, n, n * n, perms.ctypes.data_as(_dp),
+    _check(lib().pqb_perm_grad_batch(B.ctypes.data_as(_dp), count, n, ld, stride, perms.ctypes.data_as(_dp),
                                      G.ctypes.data_as(_dp)))
     return perms, G
 
@@ -403,22 +458,30 @@ def permanent_batch_device(dA_ptr: int, count: int, rows: int, cols: int, d_out_
 
 def permanent_minors(A) -> np.ndarray:
     """For a rows x (rows+1) matrix: [perm(A without column j) for j in range(rows + 1)]."""
-    M = as_matrix(A)
+    M, ld = as_rows(A)
     out = np.zeros(M.shape[1], dtype=np.complex128)
-    _check(lib().pqb_perm_minors(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], M.shape[1],
+    _check(lib().pqb_perm_minors(M.ctypes.data_as(_dp), M.shape[0], M.shape[1], ld,
                                  out.ctypes.data_as(_dp)))
     return out
 
 
 def permanent_minors_batch(A) -> np.ndarray:
     """For a (count, rows, rows+1) stack: out[b, j] = perm(A[b] without column j), one launch."""
-    M = np.ascontiguousarray(A, dtype=np.complex128)
+    M, ld, stride = _stack_strides(A)
     if M.ndim != 3:
         raise Exception("permanent_minors_batch: a (count, rows, rows + 1) array is needed")
     count, rows, cols = M.shape
     out = np.zeros((count, cols), dtype=np.complex128)
     if count == 0:
         return out
-    _check(lib().pqb_perm_minors_batch(M.ctypes.data_as(_dp), count, rows, cols, cols, rows * cols,
+    _check(lib().pqb_perm_minors_batch(M.ctypes.data_as(_dp), count, rows, cols, ld, stride,
                                        out.ctypes.data_as(_dp)))
     return out
+
+
+def probe_fp64_fma(waves_per_simd: int, iters: int = 1 << 20) -> tuple[float, float]:
+    """(ms, TFLOP/s) of the FP64 FMA issue-rate probe at that occupancy."""
+    ms = ctypes.c_double(0.0)
+    tf = ctypes.c_double(0.0)
+    _check(lib().pqb_probe_fp64_fma(waves_per_simd, iters, ctypes.byref(ms), ctypes.byref(tf)))
+    return ms.value, tf.value

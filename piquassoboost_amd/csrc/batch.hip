@@ -872,20 +872,30 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     return PQB_OK;
 }
 
-int run_plain_dd_locked(const double* A, int R, int C, int64_t ld, double* out) {
-    const std::vector<int64_t> rm((size_t)R, 1), cm((size_t)C, 1);
-    const long double S = std::ldexp(1.0L, R - 1);
-    const int range = rep_dd_range_for(C);  // the slots are the columns
-    const int64_t walkers = (int64_t)batch_grid_locked(range) * (kBlock / kWave) * rep_dd_walkers(range);
-    std::vector<TaskPlan> plans(1);
-    int st = plan_task(A, R, C, ld, rm.data(), cm.data(), choose_L(S, S, walkers, false), plans[0], true);
-    if (st) return st;
+int run_plain_dd_list_locked(const double* const* A, const int* R, const int* C, const int64_t* ld, int64_t n,
+                             double* out) {
+    std::vector<TaskPlan> plans((size_t)n);
+    for (int64_t k = 0; k < n; ++k) {
+        const std::vector<int64_t> rm((size_t)R[k], 1), cm((size_t)C[k], 1);
+        const long double S = std::ldexp(1.0L, R[k] - 1);
+        const int range = rep_dd_range_for(C[k]);  // the slots are the columns
+        const int64_t walkers = (int64_t)batch_grid_locked(range) * (kBlock / kWave) * rep_dd_walkers(range);
+        const int st = plan_task(A[k], R[k], C[k], ld[k], rm.data(), cm.data(), choose_L(S, S, walkers, false),
+                                 plans[(size_t)k], true);
+        if (st) return n > 1 ? fail(st, "matrix " + std::to_string(k) + ": " + t_err) : st;
+    }
     std::vector<double> out6;
-    st = run_tasks_locked(plans, out6, true);
+    const int st = run_tasks_locked(plans, out6, true);
     if (st) return st;
-    out[0] = out6[0];
-    out[1] = out6[1];
+    for (int64_t k = 0; k < n; ++k) {
+        out[2 * k] = out6[6 * k];
+        out[2 * k + 1] = out6[6 * k + 1];
+    }
     return PQB_OK;
+}
+
+int run_plain_dd_locked(const double* A, int R, int C, int64_t ld, double* out) {
+    return run_plain_dd_list_locked(&A, &R, &C, &ld, 1, out);
 }
 
 int batch_replay(int iters, double* ms) {

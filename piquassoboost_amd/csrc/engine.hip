@@ -187,12 +187,18 @@ PlainPlan make_plain_plan(int R, int NC, int64_t waves_resident) {
 }
 
 // Integer route eligibility.  Fills `ai` (packed R x C int32) when the matrix
-// is integer valued and real, and returns log2 of an upper bound on
-// |sum_delta ...| = 2^(R-1) |perm(A)| (or -inf when perm(A) is provably 0):
-//   |perm(A)| <= perm(|A|) <= min(prod_i r_i, prod_j c_j)  (row / column abs sums)
-// and, for entries in {-1, 0, 1}, Bregman's bound perm(|A|) <= prod_i (r_i!)^(1/r_i)
-// (also applied to the transpose).  The sweep sums modulo 2^128, so the result
-// is exact when the bound is below 2^127; column sums must also fit in int32.
+// is integer valued and real, and returns log2 of an upper bound on the sweep's
+// sum S = sum_delta (prod delta) prod_j (sum_i delta_i a_ij) (or -inf when S is
+// provably 0).  For every shape, |S| <= 2^(R-1) prod_j c_j (c_j = column abs
+// sums: each of the 2^(R-1) terms is bounded by that product; a zero column
+// makes every term 0).  Only for R == C is S = 2^(R-1) perm(A), so only there
+// do the row-sum bound perm(|A|) <= prod_i r_i and, for entries in {-1, 0, 1},
+// Bregman's bound perm(|A|) <= prod_i (r_i!)^(1/r_i) (also applied to the
+// transpose) hold; for R != C the sum is not a permanent of A (R = C + 1 gives
+// 2^(R-1) perm(A without row 0), R < C counts maps with odd row multiplicities)
+// and a row bound can be arbitrarily wrong.  The sweep sums modulo 2^128, so
+// the result is exact when the bound is below 2^127; column sums must also fit
+// in int32.
 static bool integer_matrix(const double* A, int64_t R, int64_t C, int64_t ld, std::vector<int>& ai,
                            long double* log2_bound) {
     ai.assign((size_t)(R * C), 0);
@@ -221,8 +227,11 @@ static bool integer_matrix(const double* A, int64_t R, int64_t C, int64_t ld, st
         }
         return s;
     };
-    long double lb = std::min(lg_prod(rowabs, false), lg_prod(colabs, false));
-    if (unit && R == C) lb = std::min(lb, std::min(lg_prod(rowabs, true), lg_prod(colabs, true)));
+    long double lb = lg_prod(colabs, false);
+    if (R == C) {
+        lb = std::min(lb, lg_prod(rowabs, false));
+        if (unit) lb = std::min(lb, std::min(lg_prod(rowabs, true), lg_prod(colabs, true)));
+    }
     *log2_bound = lb + (long double)(R - 1);
     return true;
 }
@@ -402,14 +411,194 @@ int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int alg
     return run_plain_locked(A, R, C, ld, out);
 }
 
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// ------------------------------------------------------------- list entry --
+//
+// Plain permanents of a list of host matrices (any mix of shapes).  Each
+// matrix gets the value pqb_perm gives it; the work is regrouped:
+//  * exact-route integer matrices: one exact sweep each (as per call);
+//  * double-double route: every matrix planned as a single call would plan it,
+//    all of them in one batched launch;
+//  * FP64 route: one launch per distinct shape, one grid row (blockIdx.y) per
+//    matrix, each row with the single-call launch plan -- so every value is
+//    bit-identical to its pqb_perm call -- with one upload of all +-2A tables,
+//    one download of all results and one synchronisation.
+
+struct ListItem {
+    int64_t idx;
+    int R, C;
+};
+
+static int list_fp64_locked(const double* const* A, const int64_t* ld, std::vector<ListItem>& items,
+                            double* out) {
+    if (items.empty()) return PQB_OK;
+    std::stable_sort(items.begin(), items.end(), [](const ListItem& a, const ListItem& b) {
+        return a.R != b.R ? a.R < b.R : a.C < b.C;
+    });
+    struct Group {
+        size_t first, count;
+        PlainPlan pl;
+        size_t part_off, m2_off;  // device byte offsets of the group's partials and tables
+    };
+    std::vector<Group> groups;
+    for (size_t k = 0; k < items.size();) {
+        size_t e = k;
+        while (e < items.size() && items[e].R == items[k].R && items[e].C == items[k].C) ++e;
+        Group g;
+        g.first = k;
+        g.count = e - k;
+        g.pl = make_plain_plan(items[k].R, items[k].C, g_eng.resident_waves_locked(items[k].C));
+        groups.push_back(g);
+        k = e;
+    }
+    const size_t n = items.size();
+    const size_t cnt_off = kCounterBytes;  // offset 0 stays the single-call counter
+    size_t off = cnt_off + round256(n * sizeof(unsigned));
+    const size_t out_off = off;
+    off += round256(n * 6 * sizeof(double));
+    for (Group& g : groups) {
+        g.part_off = off;
+        off += round256(g.count * (size_t)g.pl.blocks * sizeof(ddc));
+    }
+    size_t m2_total = 0;
+    for (Group& g : groups) {
+        g.m2_off = off + m2_total;
+        m2_total += g.count * (size_t)2 * items[g.first].R * items[g.first].C * sizeof(double2);
+    }
+    const size_t m2_base = off;
+    int st = g_eng.ensure_buffers_locked(m2_base + m2_total, m2_total + n * 6 * sizeof(double));
+    if (st) return st;
+    double* h = static_cast<double*>(g_eng.hbuf);
+    for (const Group& g : groups) {
+        const int R = items[g.first].R, C = items[g.first].C;
+        const size_t rc = (size_t)R * C;
+        double* hg = h + (g.m2_off - m2_base) / sizeof(double);
+        for (size_t k = 0; k < g.count; ++k) {
+            const ListItem& it = items[g.first + k];
+            const double* a = A[it.idx];
+            const int64_t l = ld[it.idx];
+            double* t = hg + 4 * rc * k;  // +2A then -2A, complex interleaved
+            for (int i = 0; i < R; ++i)
+                for (int j = 0; j < C; ++j) {
+                    const double re = a[2 * (i * l + j)], im = a[2 * (i * l + j) + 1];
+                    const size_t q = (size_t)i * C + j;
+                    t[2 * q] = 2.0 * re;
+                    t[2 * q + 1] = 2.0 * im;
+                    t[2 * (q + rc)] = -2.0 * re;
+                    t[2 * (q + rc) + 1] = -2.0 * im;
+                }
+        }
+    }
+    char* d = static_cast<char*>(g_eng.dbuf);
+    hipError_t e = hipMemsetAsync(d + cnt_off, 0, n * sizeof(unsigned), g_eng.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + m2_base, h, m2_total, hipMemcpyHostToDevice, g_eng.stream);
+    if (e != hipSuccess) return drain_fail(e, "list upload");
+    constexpr size_t kMaxGridY = 65535;
+    for (const Group& g : groups) {
+        const int R = items[g.first].R, C = items[g.first].C;
+        const int64_t m2n = (int64_t)2 * R * C;
+        for (size_t o = 0; o < g.count; o += kMaxGridY) {
+            const size_t nt = std::min(kMaxGridY, g.count - o);
+            PlainParams p;
+            p.M2 = reinterpret_cast<const double2*>(d + g.m2_off) + (o * (size_t)m2n);
+            p.m2_stride = m2n;
+            p.R = R;
+            p.K = g.pl.K;
+            p.B = g.pl.B;
+            p.H = g.pl.H;
+            p.partials = reinterpret_cast<ddc*>(d + g.part_off) + o * (size_t)g.pl.blocks;
+            p.counter = reinterpret_cast<unsigned*>(d + cnt_off) + g.first + o;
+            p.out = reinterpret_cast<double*>(d + out_off) + 6 * (g.first + o);
+            p.scale = std::ldexp(1.0, -(R - 1));
+            e = launch_plain(C, p, g.pl.blocks, g.pl.block, (int)nt, g_eng.stream);
+            if (e != hipSuccess) return drain_fail(e, "list sweep launch");
+        }
+    }
+    double* hout = h + m2_total / sizeof(double);
+    e = hipMemcpyAsync(hout, d + out_off, n * 6 * sizeof(double), hipMemcpyDeviceToHost, g_eng.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    if (e != hipSuccess) return drain_fail(e, "list sweep");
+    for (size_t k = 0; k < n; ++k) {
+        out[2 * items[k].idx] = hout[6 * k];
+        out[2 * items[k].idx + 1] = hout[6 * k + 1];
+    }
+    return PQB_OK;
+}
+
+int list_entry(const double* const* A, const int64_t* rows, const int64_t* cols, const int64_t* ld,
+               int64_t count, int algo, int precision, double* out) {
+    if (count < 0) return fail(PQB_ERR_ARG, "negative matrix count");
+    if (count == 0) return PQB_OK;
+    if (!A || !rows || !cols || !ld || !out) return fail(PQB_ERR_ARG, "NULL pointer");
+    if (algo < PQB_ALGO_BBFG || algo > PQB_ALGO_SIMPLE) return fail(PQB_ERR_ARG, "unknown algorithm code");
+    if (precision < PQB_PREC_DOUBLE || precision > PQB_PREC_INF)
+        return fail(PQB_ERR_ARG, "unknown precision code");
+    auto bad = [&](int64_t i, int code, const std::string& msg) {
+        return fail(code, "matrix " + std::to_string(i) + ": " + msg);
+    };
+    std::vector<ListItem> fp64, inf;
+    std::vector<std::pair<int64_t, std::vector<int>>> exact;
+    for (int64_t i = 0; i < count; ++i) {
+        if (rows[i] < 0 || cols[i] < 0) return bad(i, PQB_ERR_ARG, "negative matrix dimension");
+        if (plain_edge(algo, rows[i], cols[i], out + 2 * i)) continue;
+        if (!A[i]) return bad(i, PQB_ERR_ARG, "matrix pointer is NULL");
+        if (ld[i] < cols[i]) return bad(i, PQB_ERR_ARG, "leading dimension smaller than the number of columns");
+        if (cols[i] > PQB_MAX_COLS)
+            return bad(i, PQB_ERR_SIZE, "matrix has more than " + std::to_string(PQB_MAX_COLS) + " columns");
+        std::vector<int> ai;
+        long double lb = 0.0L;
+        const bool is_int = precision != PQB_PREC_FP64 && integer_matrix(A[i], rows[i], cols[i], ld[i], ai, &lb);
+        const bool exact_ok = is_int && exact_fits(lb);
+        if (precision == PQB_PREC_EXACT && !exact_ok)
+            return bad(i, PQB_ERR_EXACT, is_int ? "exact mode: the permanent may exceed the 128-bit range"
+                                                : "exact mode needs a real integer-valued matrix");
+        const ListItem it{i, (int)rows[i], (int)cols[i]};
+        if (exact_ok)
+            exact.emplace_back(i, std::move(ai));
+        else if (precision == PQB_PREC_INF)
+            inf.push_back(it);
+        else
+            fp64.push_back(it);
+    }
+    if (exact.empty() && inf.empty() && fp64.empty()) return PQB_OK;
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    int st = g_eng.ensure_init_locked(-1);
+    if (st) return st;
+    for (auto& ex : exact) {
+        i128 v = 0;
+        st = run_exact_locked(ex.second, (int)rows[ex.first], (int)cols[ex.first], &v);
+        if (st) return bad(ex.first, st, t_err);
+        out[2 * ex.first] = i128_to_double(v);
+        out[2 * ex.first + 1] = 0.0;
+    }
+    if (!inf.empty()) {
+        std::vector<const double*> pa;
+        std::vector<int> pr, pc;
+        std::vector<int64_t> pl;
+        for (const ListItem& it : inf) {
+            pa.push_back(A[it.idx]);
+            pr.push_back(it.R);
+            pc.push_back(it.C);
+            pl.push_back(ld[it.idx]);
+        }
+        std::vector<double> o(2 * inf.size());
+        st = run_plain_dd_list_locked(pa.data(), pr.data(), pc.data(), pl.data(), (int64_t)inf.size(), o.data());
+        if (st) return st;
+        for (size_t k = 0; k < inf.size(); ++k) {
+            out[2 * inf[k].idx] = o[2 * k];
+            out[2 * inf[k].idx + 1] = o[2 * k + 1];
+        }
+    }
+    return list_fp64_locked(A, ld, fp64, out);
+}
+
 // ----------------------------------------------------- device-pointer API --
 //
 // Workspace of the device entry points: one counter per matrix (rounded up
 // to 256 bytes), then `blocks_per_matrix` dd partials per matrix, then the
 // +-2A tables of all matrices.  For one matrix the layout is the documented
 // pqb_perm_device one (counter slot of 256 bytes, kMaxBlocks partials).
-
-static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
 size_t plain_device_bytes(int64_t count, int64_t rows, int64_t cols, int64_t blocks_per_matrix) {
     return round256((size_t)count * sizeof(unsigned)) + (size_t)count * blocks_per_matrix * sizeof(ddc) +
@@ -516,6 +705,15 @@ int pqb_perm(const double* A, int64_t rows, int64_t cols, int64_t ld, int algo, 
              double* out) {
     try {
         return plain_entry(A, rows, cols, ld, algo, precision, out, nullptr);
+    } catch (const std::exception& ex) {
+        return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}
+
+int pqb_perm_list(const double* const* A, const int64_t* rows, const int64_t* cols, const int64_t* ld,
+                  int64_t count, int algo, int precision, double* out) {
+    try {
+        return list_entry(A, rows, cols, ld, count, algo, precision, out);
     } catch (const std::exception& ex) {
         return fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
     }

@@ -49,6 +49,12 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
                 int64_t L_override);
 // Plain permanent (1 <= rows <= cols + 1) by the double-double sweep; engine lock held.
 int run_plain_dd_locked(const double* A, int R, int C, int64_t ld, double* out);
+// `n` such permanents (matrix k: A[k], R[k] x C[k], stride ld[k]), each planned as
+// run_plain_dd_locked plans it, in one batched launch; out: 2 doubles each.
+int run_plain_dd_list_locked(const double* const* A, const int* R, const int* C, const int64_t* ld, int64_t n,
+                             double* out);
+int list_entry(const double* const* A, const int64_t* rows, const int64_t* cols, const int64_t* ld,
+               int64_t count, int algo, int precision, double* out);
 
 struct Engine {
     std::mutex mu;

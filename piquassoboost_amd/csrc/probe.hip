@@ -1,0 +1,82 @@
+// FP64 issue-rate probe (measurement only): pqb_probe_fp64_fma.
+//
+// Every lane runs 8 independent chains of v_fma_f64 (no loads in the loop),
+// so the only limits are the FP64 pipe's issue rate and the clock.  Launched
+// as exactly `waves_per_simd` waves on every SIMD of the device (blocks of 4
+// waves, `waves_per_simd` blocks per CU), timed with HIP events on the engine
+// stream.  The achieved rate is what any FP64 VALU kernel can reach on this
+// device at that occupancy: the permanent sweeps are compared with it.
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "pqb_device.h"
+#include "pqb_engine.h"
+#include "pqb_perm.h"
+
+namespace pqb {
+
+__global__ __launch_bounds__(256) void fp64_fma_probe_kernel(double a, double b, int64_t iters, double* out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double x0 = 1.0 + 1e-3 * (double)(threadIdx.x & 7), x1 = x0 + 1e-4, x2 = x0 + 2e-4, x3 = x0 + 3e-4;
+    double x4 = x0 + 4e-4, x5 = x0 + 5e-4, x6 = x0 + 6e-4, x7 = x0 + 7e-4;
+    for (int64_t i = 0; i < iters; ++i) {
+        x0 = fma(x0, a, b);
+        x1 = fma(x1, a, b);
+        x2 = fma(x2, a, b);
+        x3 = fma(x3, a, b);
+        x4 = fma(x4, a, b);
+        x5 = fma(x5, a, b);
+        x6 = fma(x6, a, b);
+        x7 = fma(x7, a, b);
+    }
+    out[g] = ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7));
+}
+
+static int probe_entry(int waves_per_simd, int64_t iters, double* ms, double* tflops) {
+    if (waves_per_simd < 1 || waves_per_simd > 8) return fail(PQB_ERR_ARG, "waves_per_simd must be 1..8");
+    if (iters < 1 || iters > ((int64_t)1 << 30)) return fail(PQB_ERR_ARG, "iters must be 1..2^30");
+    if (!ms || !tflops) return fail(PQB_ERR_ARG, "NULL output pointer");
+    std::lock_guard<std::mutex> lk(g_eng.mu);
+    int st = g_eng.ensure_init_locked(-1);
+    if (st) return st;
+    const int blocks = g_eng.cus * waves_per_simd;  // 4 waves per block: one per SIMD
+    const size_t out_bytes = (size_t)blocks * kBlock * sizeof(double);
+    st = g_eng.ensure_buffers_locked(kCounterBytes + out_bytes, 0);
+    if (st) return st;
+    double* dout = reinterpret_cast<double*>(static_cast<char*>(g_eng.dbuf) + kCounterBytes);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // one untimed launch (code load, clock ramp), then the timed one
+    for (int rep = 0; e == hipSuccess && rep < 2; ++rep) {
+        if (rep == 1) e = hipEventRecord(e0, g_eng.stream);
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(fp64_fma_probe_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, g_eng.stream,
+                           1.0 - 1e-9, 1e-9, iters, dout);
+        e = hipGetLastError();
+        if (e == hipSuccess && rep == 1) e = hipEventRecord(e1, g_eng.stream);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float f = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "fp64 probe");
+    *ms = f;
+    const double flops = (double)blocks * kBlock * (double)iters * 8.0 * 2.0;
+    *tflops = flops / ((double)f * 1e-3) / 1e12;
+    return PQB_OK;
+}
+
+}  // namespace pqb
+
+extern "C" int pqb_probe_fp64_fma(int waves_per_simd, int64_t iters, double* ms, double* tflops) {
+    try {
+        return pqb::probe_entry(waves_per_simd, iters, ms, tflops);
+    } catch (const std::exception& ex) {
+        return pqb::fail(PQB_ERR_DEVICE, std::string("internal error: ") + ex.what());
+    }
+}

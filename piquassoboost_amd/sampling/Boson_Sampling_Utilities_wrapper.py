@@ -124,8 +124,9 @@ class GlynnPermanentCalculator_wrapper:
         GlynnPermanentCalculator_Wrapper.hpp:204-293)."""
         algo, prec = PLAIN_LIBS[self.lib]
         if isinstance(self._matrix, list):
-            mats = [C.as_matrix(m) for m in self._matrix]
-            return [_run(C.permanent, m, algo, prec) for m in mats]
+            # one call for the whole list (the reference loops over the matrices,
+            # GlynnPermanentCalculator_Wrapper.hpp:210-237): one launch per shape
+            return _run(C.permanent_list, self._matrix, algo, prec)
         return _run(C.permanent, self._matrix, algo, prec)
 
 

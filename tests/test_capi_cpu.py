@@ -176,6 +176,104 @@ def test_exact_route_bounds():
     assert not C.exact_bound(np.array([[1j, 1.0], [1.0, 1.0]]))[0]
 
 
+def _true_log2_sum(A):
+    """log2 |sum_delta (prod delta) prod_j (sum_i delta_i a_ij)| in Python integers."""
+    import math
+
+    from exact_sum import exact_sum
+
+    tr, ti, k = exact_sum(np.asarray(A, dtype=np.complex128))
+    assert ti == 0
+    return (math.log2(abs(tr)) + k + (A.shape[0] - 1)) if tr else float("-inf")
+
+
+@pytest.mark.parametrize("shape", [(1, 40), (2, 30), (3, 2), (4, 3), (3, 7), (6, 6), (5, 4)])
+def test_exact_route_bound_holds_for_every_shape(shape):
+    """The exact route's bound must dominate the sweep's sum for every shape it
+    accepts: for rows != cols the sum is not a permanent of the matrix, so only
+    the column abs-sum bound applies (rows = cols + 1 gives 2^(R-1) perm(A
+    without row 0); rows < cols counts maps with odd row multiplicities)."""
+    r, c = shape
+    rng = np.random.default_rng(7700 + 10 * r + c)
+    for A in (np.full(shape, 1000.0), np.full(shape, 50.0), rng.integers(-3, 4, size=shape).astype(float)):
+        A = A.copy()
+        if r == c + 1:
+            A[0] = 0.0  # a zero anchor row does not make the sum zero
+        ok, lb = C.exact_bound(A)
+        true = _true_log2_sum(A)
+        assert true <= lb + 1e-9, (shape, A[0, 0], true, lb)
+        if ok:
+            assert lb < 126.5
+
+
+def test_exact_route_rejects_wide_rectangles():
+    """1 x 40 of 1000s (sum ~2^398), 2 x 30 of 50s (~2^199): too large for 128 bits;
+    rows = cols + 1 with a zero row 0: not provably zero (perm of the other rows)."""
+    assert not C.exact_bound(np.full((1, 40), 1000.0))[0]
+    assert not C.exact_bound(np.full((2, 30), 50.0))[0]
+    Z = np.array([[0.0, 0.0], [1000.0, 7.0], [3.0, 1000.0]])
+    ok, lb = C.exact_bound(Z)
+    assert ok and np.isfinite(lb)
+    assert abs(_true_log2_sum(Z) - (np.log2(1000.0 * 1000.0 + 21.0) + 2)) < 1e-9
+
+
+def _wrap128(v):
+    v &= (1 << 128) - 1
+    return v - (1 << 128) if v >> 127 else v
+
+
+def _grouped_product(c, G):
+    """The exact sweep's product of column sums: G sums at a time multiplied in
+    int64, each partial product folded into a 128-bit word by one 64 x 64 -> 128
+    product of the low word, one 64-bit product of the high word and a negation
+    for a negative factor (perm_plain.hip, mul_u128_s64)."""
+    M64, M128 = (1 << 64) - 1, (1 << 128) - 1
+    pr = None
+    for j0 in range(0, len(c), G):
+        q = 1
+        for x in c[j0:j0 + G]:
+            q *= x
+        assert -(1 << 63) <= q < (1 << 63), "group product leaves int64"
+        if pr is None:
+            pr = q & M128
+            continue
+        m = (-q) & M64 if q < 0 else q
+        lo = (pr & M64) * m
+        hi = (((pr >> 64) * m) + (lo >> 64)) & M64
+        r = ((hi << 64) | (lo & M64)) & M128
+        pr = (-r) & M128 if q < 0 else r
+    return _wrap128(pr)
+
+
+@pytest.mark.parametrize("G,cmax", [(13, 28), (6, 1400), (2, (1 << 30) - 1)])
+def test_grouped_product_matches_128bit_product(G, cmax):
+    """At each threshold the host picks (largest column abs sum 28 -> G = 13,
+    1400 -> 6, below 2^30 -> 2), every group's product fits int64 and the folded
+    result equals the product modulo 2^128; column sums at +-cmax exactly included."""
+    rng = np.random.default_rng(7800 + G)
+    assert cmax ** G < (1 << 63)
+    for n in (1, G - 1, G, G + 1, 2 * G + 3, 26, 40):
+        if n < 1:
+            continue
+        for trial in range(400):
+            if trial % 4 == 0:
+                c = [int(cmax) * int(s) for s in rng.choice([-1, 1], size=n)]
+            else:
+                c = [int(x) for x in rng.integers(-cmax, cmax + 1, size=n)]
+            exact = 1
+            for x in c:
+                exact *= x
+            assert _grouped_product(c, G) == _wrap128(exact), (G, n, c)
+
+
+def test_grouped_product_thresholds_are_tight():
+    """The next column abs sum above each threshold would overflow int64 for the
+    larger group (29^13 > 2^63) -- the reason the host switches group size there."""
+    assert 28 ** 13 < (1 << 63) <= 29 ** 13
+    assert 1400 ** 6 < (1 << 63)
+    assert ((1 << 30) - 1) ** 2 < (1 << 63)
+
+
 def test_repeated_plan_host_only():
     """Task planning needs no device: anchor, orientation, digit split."""
     U = np.eye(6, dtype=complex) + 0.1
@@ -321,3 +419,26 @@ def test_batch_validation_from_concurrent_host_threads():
 
     with ThreadPoolExecutor(8) as ex:
         assert list(ex.map(work, range(8))) == [None] * 8
+
+
+def test_row_stride_binding_host_only():
+    """Row-strided complex128 arrays go to the C ABI in place with their row
+    stride as ld; anything else is copied to a packed array (ld = cols)."""
+    P = np.arange(40 * 40, dtype=np.float64).reshape(40, 40) * (1 + 1j)
+    M, ld = C.as_rows(P[3:23, 5:25])
+    assert ld == 40 and M.ctypes.data == P.ctypes.data + 16 * (3 * 40 + 5)
+    M, ld = C.as_rows(P[::2, :10])  # every other row: stride 80
+    assert ld == 80
+    M, ld = C.as_rows(P[:, ::2])  # column step: copied
+    assert ld == 20 and M.flags.c_contiguous
+    M, ld = C.as_rows(P.T)  # transposed: copied
+    assert ld == 40 and M.flags.c_contiguous and np.array_equal(M, P.T)
+    M, ld = C.as_rows(P[::-1])  # negative row stride: copied
+    assert ld == 40 and np.array_equal(M, P[::-1])
+    M, ld = C.as_rows(P.real[:4, :4])  # float64: converted
+    assert ld == 4 and M.dtype == np.complex128
+    S = np.zeros((3, 12, 16), dtype=np.complex128)
+    B, ld, stride = C._stack_strides(S[:, :10, :11])
+    assert (ld, stride) == (16, 192) and B.ctypes.data == S.ctypes.data
+    B, ld, stride = C._stack_strides(S[:, :, ::2])
+    assert (ld, stride) == (8, 96) and B.flags.c_contiguous

@@ -7,6 +7,7 @@ import pytest
 
 import oracle
 from conftest import GOLDEN, random_complex, rel_err
+from exact_sum import exact_bbfg
 from piquassoboost_amd import _capi as C
 from piquassoboost_amd.sampling import Boson_Sampling_Utilities as BSU
 
@@ -69,6 +70,89 @@ def test_list_batched_plain():
     assert isinstance(res, list) and len(res) == 4
     for m, v in zip(mats, res):
         assert rel_err(v, oracle.bbfg(m, "dd")) < TOL_SMALL
+
+
+def _list_mix(seed):
+    """A list of uneven plain matrices: square n = 1..24 (several of each shape),
+    rectangular rows = cols + 1 and rows < cols, decided shapes (empty, rows >=
+    cols + 2), integer matrices (exact route) and row slices of a wider array."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    for n in list(range(1, 17)) * 3 + [18, 20, 20, 22, 24]:
+        mats.append(random_complex(rng, n, n))
+    for r, c in [(5, 4), (9, 8), (2, 5), (4, 9), (7, 12), (5, 4), (0, 3), (6, 3), (1, 1)]:
+        mats.append(random_complex(rng, r, c) if r and c else np.zeros((r, c), np.complex128))
+    for n in (6, 10, 12):
+        mats.append(rng.integers(-2, 3, size=(n, n)).astype(np.complex128))
+    wide = random_complex(rng, 14, 30)
+    mats += [wide[:, 2:16], wide[1:13, 5:17]]
+    order = rng.permutation(len(mats))
+    return [mats[i] for i in order]
+
+
+@pytest.mark.parametrize("lib", [0, 6, 7, 8, 10, 11])
+def test_list_surface_equals_per_call_bitwise(lib):
+    """GlynnPermanentCalculator_Wrapper.hpp:204-293: a list of matrices in one call
+    (pqb_perm_list, one launch per shape on the FP64 route) gives, for every matrix,
+    the value of the per-matrix call bit for bit, and the oracle's value within the
+    usual bar."""
+    from piquassoboost_amd.sampling.Boson_Sampling_Utilities_wrapper import PLAIN_LIBS
+
+    mats = _list_mix(8100 + lib)
+    algo, prec = PLAIN_LIBS[lib]
+    got = BSU.GlynnPermanentCalculator_wrapper(mats, lib).calculate()
+    assert isinstance(got, list) and len(got) == len(mats)
+    worst = 0.0
+    for k, (m, v) in enumerate(zip(mats, got)):
+        one = C.permanent(m, algo, prec)
+        assert v == one, (k, m.shape, v, one)
+        r, c = m.shape
+        if r == 0 or c == 0 or (algo == C.ALGO_SIMPLE and r != c) or r >= c + 2:
+            continue
+        ref = oracle.bbfg(m, "dd")
+        scale = abs(ref) if r == c else max(abs(ref), np.prod(np.abs(m).sum(axis=0)) * 1e-3)
+        err = abs(v - ref) / scale
+        worst = max(worst, err)
+        assert err < TOL_SMALL, (k, m.shape, v, ref)
+    print(f"lib {lib}: {len(mats)} matrices, worst rel_err vs oracle {worst:.2e}", flush=True)
+
+
+def test_list_surface_inf_and_exact():
+    """The double-double route (lib 1) in one batched launch, and the exact route of
+    integer matrices inside a list: equal to the per-call values, and the
+    double-double ones to the correctly rounded exact sum."""
+    rng = np.random.default_rng(8200)
+    mats = [random_complex(rng, n, n) for n in (3, 7, 9, 9, 11)] + [random_complex(rng, 5, 4)]
+    mats.append(rng.integers(-3, 4, size=(8, 8)).astype(np.complex128))
+    got = BSU.GlynnPermanentCalculator_wrapper(mats, 1).calculate()
+    for m, v in zip(mats, got):
+        assert v == C.permanent(m, C.ALGO_BBFG, C.PREC_INF)
+        assert v == exact_bbfg(m), (m.shape, v)
+    ints = [rng.integers(-2, 3, size=(n, n)).astype(float) for n in (5, 9, 13)]
+    vals = C.permanent_list(ints, C.ALGO_BBFG, C.PREC_EXACT)
+    for m, v in zip(ints, vals):
+        assert v == complex(float(oracle.int_permanent(m.astype(np.int64))), 0.0)
+
+
+def test_list_surface_errors_name_the_matrix():
+    mats = [np.eye(3), np.ones((2, 41)), np.eye(2)]
+    with pytest.raises(Exception, match="matrix 1: matrix has more than 40 columns"):
+        BSU.GlynnPermanentCalculator_wrapper(mats, 7).calculate()
+    with pytest.raises(C.EngineError, match="matrix 1: exact mode needs a real integer-valued matrix"):
+        C.permanent_list([np.eye(3), np.eye(3) * 0.5], C.ALGO_BBFG, C.PREC_EXACT)
+    assert C.permanent_list([]) == []
+
+
+def test_list_surface_large_uniform_batch():
+    """1000 matrices of n = 12 (one launch, 1000 grid rows) and 64 of n = 20: each
+    bit-identical to its own call."""
+    rng = np.random.default_rng(8300)
+    mats = [random_complex(rng, 12, 12) for _ in range(1000)] + [random_complex(rng, 20, 20) for _ in range(64)]
+    got = C.permanent_list(mats, C.ALGO_BBFG, C.PREC_FP64)
+    for k in list(range(0, 1000, 37)) + list(range(1000, 1064, 7)):
+        assert got[k] == C.permanent(mats[k], C.ALGO_BBFG, C.PREC_FP64), k
+    for k in range(0, 1064, 131):
+        assert rel_err(got[k], oracle.bbfg(mats[k], "dd")) < TOL_SMALL, k
 
 
 def test_haar_n20(haar_blocks):
@@ -187,6 +271,85 @@ def derangements(n):
     for k in range(2, n + 1):
         a, b = b, (k - 1) * (a + b)
     return b
+
+
+def _with_column_abs_sum(rng, n, lo, hi, target):
+    """n x n integers in [lo, hi] whose largest column abs sum is exactly `target`."""
+    while True:
+        A = rng.integers(lo, hi + 1, size=(n, n))
+        cs = np.abs(A).sum(axis=0)
+        if cs.max() > target:
+            continue
+        # raise column 0 to the target by growing entries away from zero
+        need = target - cs[0]
+        for i in range(n):
+            if need == 0:
+                break
+            room = max(abs(lo), abs(hi)) - abs(A[i, 0])
+            step = min(room, need)
+            A[i, 0] = A[i, 0] + step if A[i, 0] >= 0 else A[i, 0] - step
+            need -= step
+        if need == 0 and np.abs(A).sum(axis=0).max() == target:
+            return A
+
+
+# (n, entry range, largest column abs sum): the host multiplies 13 column sums at a
+# time in int64 up to a column abs sum of 28, 6 at a time up to 1400, else 2
+EXACT_GROUPS = [(14, (-2, 2), 28), (14, (-3, 3), 29), (10, (-200, 200), 1400), (10, (-200, 200), 1401),
+                (10, (-100, 100), None), (6, (-100000, 100000), None), (4, (-(1 << 28) + 1, (1 << 28) - 1), None)]
+
+
+@pytest.mark.parametrize("case", range(len(EXACT_GROUPS)))
+def test_exact_group_thresholds(case):
+    """The exact sweep at each grouped-product variant (G = 13, 6, 2) and at the
+    column abs sums where the host switches (28 / 29, 1400 / 1401): equal to the
+    oracle's 128-bit Ryser permanent."""
+    n, (lo, hi), target = EXACT_GROUPS[case]
+    rng = np.random.default_rng(7900 + case)
+    A = _with_column_abs_sum(rng, n, lo, hi, target) if target else rng.integers(lo, hi + 1, size=(n, n))
+    cmax = int(np.abs(A).sum(axis=0).max())
+    ok, lb = C.exact_bound(A.astype(float))
+    if not ok:
+        pytest.fail(f"case {case}: outside the exact bound (log2 {lb:.1f})")
+    exact = oracle.int_permanent(A.astype(np.int64))
+    v, d = C.permanent_exact(A.astype(float))
+    print(f"n={n} cmax={cmax} group={13 if cmax <= 28 else (6 if cmax <= 1400 else 2)} "
+          f"log2 bound={lb:.1f} perm={exact}", flush=True)
+    assert v == exact
+    assert d == float(exact)
+    got = BSU.BBFGPermanentDouble(A.astype(float)).calculate()
+    assert got.real == float(exact) and got.imag == 0.0
+
+
+@pytest.mark.parametrize("shape", [(3, 2), (5, 4), (9, 8), (2, 5), (4, 9), (7, 12), (1, 6), (2, 30)])
+def test_exact_rectangular_integer_default_route(shape):
+    """Rectangular integer matrices through the default precision (which routes
+    provably representable integer sums to the exact sweep): equal to the BBFG
+    sum evaluated exactly in Python integers and rounded once.  rows = cols + 1
+    cases have a zero row 0 (the sum is then perm of the other rows, not 0)."""
+    r, c = shape
+    rng = np.random.default_rng(8000 + 10 * r + c)
+    A = rng.integers(-3, 4, size=shape).astype(float)
+    if r == c + 1:
+        A[0] = 0.0
+    ok, lb = C.exact_bound(A)
+    ref = exact_bbfg(A)
+    got = C.permanent(A, C.ALGO_BBFG, C.PREC_DOUBLE)
+    print(f"shape={shape} exact route={ok} (log2 bound {lb:.1f}) got={got!r} exact={ref!r}", flush=True)
+    assert ok
+    assert got == ref
+    if r == c + 1:
+        assert ref == complex(float(oracle.int_permanent(A[1:].astype(np.int64))), 0.0)
+
+
+def test_exact_wide_rectangles_fall_back_to_fp64():
+    """Sums beyond the 128-bit range (1 x 40 of 1000s, 2 x 30 of 50s) must not take
+    the exact route; the FP64 sweep's value is checked against the exact sum."""
+    for A in (np.full((1, 40), 1000.0), np.full((2, 30), 50.0)):
+        assert not C.exact_bound(A)[0]
+        ref = exact_bbfg(A)
+        got = C.permanent(A, C.ALGO_BBFG, C.PREC_DOUBLE)
+        assert rel_err(got, ref) < 1e-13, (A.shape, got, ref)
 
 
 def test_exact_derangements():
