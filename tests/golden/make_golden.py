@@ -177,7 +177,75 @@ def seeding():
     print("seeding.json")
 
 
-STEPS = {"kat": kat, "haar": haar_blocks, "bernoulli": bernoulli, "c3": repeated_c3, "seeding": seeding}
+def uneven4096():
+    """The batched workload in full: 4096 transitions of the seed-1005 100-mode Haar
+    unitary (n ~ U{8..26} photons in distinct input modes, output multiplicities
+    {1,2,3} with p = .7/.2/.1, as utilities.uneven_batch draws them).  Stored: the
+    unitary itself, each task's input modes and (output mode, multiplicity) lists
+    (padded with -1 / 0), and the oracle's double-double value of every task
+    (restated BBFGPermanentCalculatorRepeated on the compact matrix, as
+    BatchedPermanentCalculator.cpp:118-145 evaluates each pair)."""
+    from piquassoboost_amd.sampling.utilities import adapt_interferometer, uneven_batch
+
+    U, pairs = uneven_batch(seed=1005, count=4096)
+    count = len(pairs)
+    in_modes = np.full((count, 26), -1, dtype=np.int8)
+    out_modes = np.full((count, 26), -1, dtype=np.int8)
+    out_mult = np.zeros((count, 26), dtype=np.int8)
+    vals = np.zeros((count, 4), dtype=np.float64)
+    t = time.time()
+    for k, (i, o) in enumerate(pairs):
+        im = np.flatnonzero(i)
+        om = np.flatnonzero(o)
+        in_modes[k, :im.size] = im
+        out_modes[k, :om.size] = om
+        out_mult[k, :om.size] = o[om]
+        A = adapt_interferometer(U, i, o)
+        _, vp = oracle.bbfg_repeated(A, i[im], o[om], "dd", full=True)
+        vals[k] = vp
+        if k % 512 == 511:
+            print("uneven4096", k + 1, f"{time.time() - t:.0f}s", flush=True)
+    np.savez_compressed(HERE / "uneven4096.npz", U=U, in_modes=in_modes, out_modes=out_modes, out_mult=out_mult,
+                        value_dd=vals)
+
+
+def grad_minors():
+    """Minors of the gradient workload's blocks (block b: top-left n x n of the
+    50-mode Haar unitary of seed 1006 + b): for all 8 n=26 blocks the row-0 and
+    column-0 minors (d perm / d a_0j and d perm / d a_i0: 51 n=25 permanents per
+    block), for the first 16 n=20 blocks every minor (400 n=19 permanents per block),
+    each the oracle's double-double restated reference sweep."""
+    from piquassoboost_amd.sampling.utilities import gradient_blocks
+
+    blocks, seeds = gradient_blocks()
+    out = {}
+    t = time.time()
+    B26 = blocks[26]
+    row0 = np.zeros((B26.shape[0], 26, 4))
+    col0 = np.zeros((B26.shape[0], 26, 4))
+    for b in range(B26.shape[0]):
+        A = B26[b]
+        for j in range(26):
+            row0[b, j] = oracle.bbfg(np.ascontiguousarray(np.delete(np.delete(A, 0, 0), j, 1)), "dd", full=True)[1]
+        col0[b, 0] = row0[b, 0]
+        for i in range(1, 26):
+            col0[b, i] = oracle.bbfg(np.ascontiguousarray(np.delete(np.delete(A, i, 0), 0, 1)), "dd", full=True)[1]
+        print("grad_minors n=26 block", b, f"{time.time() - t:.0f}s", flush=True)
+    B20 = blocks[20][:16]
+    full20 = np.zeros((B20.shape[0], 20, 20, 4))
+    for b in range(B20.shape[0]):
+        A = B20[b]
+        for i in range(20):
+            for j in range(20):
+                full20[b, i, j] = oracle.bbfg(np.ascontiguousarray(np.delete(np.delete(A, i, 0), j, 1)), "dd",
+                                              full=True)[1]
+    print("grad_minors n=20", f"{time.time() - t:.0f}s", flush=True)
+    out.update(b26=B26, b26_row0_dd=row0, b26_col0_dd=col0, b20=B20, b20_minors_dd=full20)
+    np.savez_compressed(HERE / "grad_minors.npz", **out)
+
+
+STEPS = {"kat": kat, "haar": haar_blocks, "bernoulli": bernoulli, "c3": repeated_c3, "seeding": seeding,
+         "uneven4096": uneven4096, "grad_minors": grad_minors}
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import rel_err
+from conftest import GOLDEN, rel_err
 from piquassoboost_amd import _capi as C
 from piquassoboost_amd.sampling.utilities import adapt_interferometer, uneven_batch
 
@@ -37,6 +37,60 @@ def test_uneven_batch_against_oracle(uneven_batch4096):
         worst = max(worst, err)
         assert err < 1e-10, (int(k), int(cols.sum()), err)
     print(f"uneven batch, sample of 40: worst rel_err={worst:.3e}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def uneven_fixture():
+    """The stored workload (unitary, modes, multiplicities) and the oracle's
+    double-double value of every task (tests/golden/make_golden.py)."""
+    d = np.load(GOLDEN / "uneven4096.npz")
+    U = d["U"]
+    m = U.shape[0]
+    count = d["in_modes"].shape[0]
+    ins = np.zeros((count, m), dtype=np.int64)
+    outs = np.zeros((count, m), dtype=np.int64)
+    for k in range(count):
+        im = d["in_modes"][k]
+        im = im[im >= 0]
+        ins[k, im] = 1
+        om = d["out_modes"][k]
+        sel = om >= 0
+        outs[k, om[sel]] = d["out_mult"][k][sel]
+    v = d["value_dd"]
+    ref = (v[:, 0] + v[:, 1]) + 1j * (v[:, 2] + v[:, 3])
+    return U, ins, outs, ref
+
+
+def _worst(got, ref):
+    err = np.abs(np.asarray(got) - ref) / np.abs(ref)
+    return err, int(np.argmax(err))
+
+
+def test_uneven_batch_every_result_vs_fixture(uneven_fixture):
+    """Every one of the 4096 values of the batched workload, through both batched
+    entry points (the task list, pqb_perm_batch, and the unitary with state
+    arrays, pqb_perm_batch_states), within 1e-10 of the oracle's double-double
+    value (BatchedPermanentCalculator.cpp:118-145 evaluates each pair with the
+    reference calculator)."""
+    U, ins, outs, ref = uneven_fixture
+    tasks = [(adapt_interferometer(U, i, o), o[o > 0], i[i > 0]) for i, o in zip(ins, outs)]
+    for name, got in (("pqb_perm_batch", C.permanent_batch(tasks)),
+                      ("pqb_perm_batch_states", C.permanent_batch_states(U, ins, outs))):
+        err, k = _worst(got, ref)
+        photons = int(ins[k].sum())
+        print(f"{name}: 4096 results, worst rel_err={err[k]:.3e} (task {k}, {photons} photons), "
+              f"median {np.median(err):.2e}", flush=True)
+        assert err[k] < 1e-10, (name, k)
+
+
+def test_uneven_batch_every_result_per_call_vs_fixture(uneven_fixture):
+    """The same 4096 tasks one call each (pqb_perm_repeated), each within 1e-10 of the
+    oracle: the per-call path the batch is measured against."""
+    U, ins, outs, ref = uneven_fixture
+    got = [C.permanent_repeated(adapt_interferometer(U, i, o), o[o > 0], i[i > 0]) for i, o in zip(ins, outs)]
+    err, k = _worst(got, ref)
+    print(f"pqb_perm_repeated per call: worst rel_err={err[k]:.3e} (task {k})", flush=True)
+    assert err[k] < 1e-10
 
 
 def test_uneven_batch_full_mode_form_matches_compact(uneven_batch4096):

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import random_complex, rel_err
+from conftest import GOLDEN, random_complex, rel_err
 from piquassoboost_amd import _capi as C
 from piquassoboost_amd.sampling.utilities import gradient_blocks
 
@@ -81,6 +81,59 @@ def test_grad_n20_all_minors(grad_blocks):
                 assert e < 1e-9, (b, i, j, e)
         assert rel_err(perms[b], perm_oracle(A, "dd")) < 1e-12
     print(f"grad n=20 minors: worst rel_err={worst:.3e}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def minors_fixture():
+    d = np.load(GOLDEN / "grad_minors.npz")
+
+    def c(a):
+        return (a[..., 0] + a[..., 1]) + 1j * (a[..., 2] + a[..., 3])
+
+    return dict(b26=d["b26"], row0=c(d["b26_row0_dd"]), col0=c(d["b26_col0_dd"]), b20=d["b20"],
+                m20=c(d["b20_minors_dd"]))
+
+
+def test_grad_workload_n26_row_col_minors_vs_fixture(grad_blocks, minors_fixture):
+    """All 8 n=26 blocks of the gradient workload in one device launch
+    (pqb_perm_grad_device, as the benchmark runs it): the row-0 and column-0
+    gradients (51 n=25 minors per block) within 1e-9 of the oracle's double-double
+    minor permanents; the blocks are the stored ones."""
+    import torch
+
+    f = minors_fixture
+    assert np.array_equal(f["b26"], grad_blocks[26])
+    B = f["b26"]
+    count, n = B.shape[0], 26
+    dA = torch.from_numpy(B).cuda()
+    dp = torch.empty(count, dtype=torch.complex128, device="cuda")
+    dG = torch.empty_like(dA)
+    wb = C.grad_workspace_bytes(count, n)
+    work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+    C.permanent_grad_device(dA.data_ptr(), count, n, dp.data_ptr(), dG.data_ptr(), work.data_ptr(), wb,
+                            torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    G = dG.cpu().numpy()
+    err_r = np.abs(G[:, 0, :] - f["row0"]) / np.abs(f["row0"])
+    err_c = np.abs(G[:, :, 0] - f["col0"]) / np.abs(f["col0"])
+    worst = max(err_r.max(), err_c.max())
+    print(f"grad n=26, 8 blocks x 51 minors: worst rel_err={worst:.3e} (median {np.median(err_r):.2e})",
+          flush=True)
+    assert worst < 1e-9
+    # host batch entry on the same blocks: same values to the bit (same plan)
+    _, Gh = C.permanent_grad_batch(B)
+    assert np.array_equal(Gh, G)
+
+
+def test_grad_workload_n20_all_minors_vs_fixture(minors_fixture):
+    """16 n=20 blocks in one batched call: every one of the 400 minors per block
+    within 1e-9 of the oracle's double-double value."""
+    f = minors_fixture
+    _, G = C.permanent_grad_batch(f["b20"])
+    err = np.abs(G - f["m20"]) / np.abs(f["m20"])
+    print(f"grad n=20, 16 blocks x 400 minors: worst rel_err={err.max():.3e} (median {np.median(err):.2e})",
+          flush=True)
+    assert err.max() < 1e-9
 
 
 def test_grad_n26_sampled_minors(grad_blocks):
