@@ -150,9 +150,9 @@ def repeated_c3():
 def seeding():
     """Seeded boson-sampling runs of unittests/test_seeding.py (data of the reference's
     tests): the 5-mode interferometer (:30-49), input state [2,1,1,0,1], seed 123, 20 shots;
-    expected samples of the lossless run (:63-84) and of the uniformly lossy one with
-    singular values 0.9 (:134-155).  The non-uniform lossy run (:97-118) is not stored: its
-    2m-mode dilation is built by piquasso, which is not part of the reference."""
+    expected samples of the lossless run (:63-84), of the lossy one with singular values
+    0.9 .. 0.5 (:88-125, run through the 2m-mode dilation) and of the uniformly lossy one
+    with singular values 0.9 (:134-155)."""
     U = [[-0.17959207 - 0.29175972j, -0.64550941 - 0.02897055j, -0.023922 - 0.38854671j,
           -0.07908932 + 0.35617293j, -0.39779191 + 0.14902272j],
          [-0.36896208 + 0.19468375j, -0.11545557 + 0.20434514j, 0.25548079 + 0.05220164j,
@@ -171,8 +171,13 @@ def seeding():
                (0, 0, 5, 0, 0), (1, 0, 1, 0, 2), (0, 1, 1, 1, 0), (0, 0, 2, 3, 0), (0, 0, 1, 2, 0),
                (0, 0, 0, 1, 2), (0, 0, 1, 2, 1), (2, 1, 1, 0, 1), (2, 0, 2, 0, 0), (0, 2, 1, 0, 0),
                (1, 0, 0, 2, 1), (3, 0, 0, 0, 1), (2, 1, 1, 0, 1), (0, 0, 1, 2, 2), (2, 1, 1, 0, 1)]
+    nonuniform = [(0, 3, 1, 0, 0), (0, 0, 0, 0, 4), (0, 1, 0, 0, 2), (0, 0, 2, 0, 1), (0, 2, 1, 0, 0),
+                  (0, 0, 1, 0, 1), (1, 0, 0, 1, 2), (0, 0, 0, 1, 2), (0, 0, 0, 1, 3), (2, 0, 0, 1, 1),
+                  (0, 1, 0, 0, 0), (0, 1, 0, 1, 0), (0, 0, 0, 1, 1), (0, 0, 1, 2, 0), (0, 0, 2, 0, 0),
+                  (3, 0, 0, 0, 0), (0, 1, 1, 0, 2), (0, 0, 1, 1, 0), (0, 0, 0, 0, 0), (0, 0, 1, 1, 0)]
     data = dict(U=[[_c(v) for v in row] for row in U], input_state=[2, 1, 1, 0, 1], seed=123, shots=20,
-                uniform_singular_value=0.9, lossless=lossless, uniform_losses=uniform)
+                uniform_singular_value=0.9, lossless=lossless, uniform_losses=uniform,
+                nonuniform_singular_values=[0.9, 0.8, 0.7, 0.6, 0.5], nonuniform_losses=nonuniform)
     (HERE / "seeding.json").write_text(json.dumps(data, indent=1))
     print("seeding.json")
 

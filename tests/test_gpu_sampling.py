@@ -56,6 +56,23 @@ def test_uniform_losses_seeded(seeding):
     assert Counter(gpu) == Counter(tuple(x) for x in d["uniform_losses"])
 
 
+def test_nonuniform_losses_seeded(seeding):
+    """unittests/test_seeding.py:88-125 on the GPU: the 2m-mode dilation of the lossy
+    interferometer (sampling/calculations.py), its permanents from the engine (lock-step
+    column minors), the same samples as the oracle-driven run, and the reference test's
+    multiset."""
+    from piquassoboost_amd.sampling import calculations
+
+    d = seeding
+    U = d["U"]
+    lossy = U @ np.diag(d["nonuniform_singular_values"]) @ U @ U.T
+    gpu = [tuple(int(x) for x in v) for v in calculations.sample(lossy, d["input_state"], d["shots"], d["seed"])]
+    assert Counter(gpu) == Counter(tuple(x) for x in d["nonuniform_losses"])
+    cpu = [tuple(int(x) for x in v)
+           for v in calculations.sample(lossy, d["input_state"], d["shots"], d["seed"], permanent_batch=oracle_batch)]
+    assert gpu == cpu
+
+
 def test_compute_pmf_large_step():
     """14 photons in 40 modes, collisions in input and sample: the column-minors sweep,
     the batched per-mode permanents and the oracle agree."""

@@ -156,3 +156,52 @@ def test_lockstep_uniform_losses_reproduces_the_sequential_loop(seeding, oracle_
     assert a == b
     assert len(set(sum(x) for x in a)) > 1  # the photon counts do differ
     assert Counter(a) == Counter(tuple(x) for x in d["uniform_losses"])
+
+
+def test_nonuniform_losses_seeded_samples(seeding):
+    """unittests/test_seeding.py:88-125: LossyInterferometer U diag(0.9..0.5) U U^T, input
+    [2,1,1,0,1], seed 123, 20 shots.  Non-uniform singular values: the 2m-mode unitary
+    dilation (piquasso 7.2.1's, restated in sampling/calculations.py), the lossless
+    strategy on it with the input padded by empty modes, the samples cut to 5 modes
+    (calculations.py:121-145); the 20-sample multiset equals the reference test's."""
+    from piquassoboost_amd.sampling import calculations
+
+    d = seeding
+    U = d["U"]
+    lossy = U @ np.diag(d["nonuniform_singular_values"]) @ U @ U.T
+    got = calculations.sample(lossy, d["input_state"], d["shots"], d["seed"], permanent_batch=oracle_batch)
+    assert got.shape == (20, 5)
+    assert Counter(map(tuple, got.tolist())) == Counter(tuple(x) for x in d["nonuniform_losses"])
+    # the reference's Branch frequencies
+    res = calculations.particle_number_measurement(lossy, d["input_state"], d["shots"], d["seed"],
+                                                   permanent_batch=oracle_batch)
+    assert sum(f for _, f in res) == 1
+    assert {o: f * 20 for o, f in res} == dict(Counter(tuple(x) for x in d["nonuniform_losses"]))
+
+
+def test_dilation_is_unitary_on_the_occupied_modes(seeding):
+    """The dilation's first m columns are orthonormal and its top-left block is the lossy
+    matrix: the marginal on the first m output modes is the lossy process's."""
+    from piquassoboost_amd.sampling import calculations
+
+    d = seeding
+    U = d["U"]
+    lossy = U @ np.diag(d["nonuniform_singular_values"]) @ U @ U.T
+    W = calculations.prepare_interferometer_matrix_in_expanded_space(np.linalg.svd(lossy))
+    assert W.shape == (10, 10)
+    assert np.abs(W[:5, :5] - lossy).max() < 1e-14
+    assert np.abs(W[:, :5].conj().T @ W[:, :5] - np.eye(5)).max() < 1e-14
+
+
+def test_calculations_branches_match_the_strategies(seeding):
+    """Lossless and uniform-loss inputs go to the same strategies the existing fixtures
+    pin (same samples)."""
+    from piquassoboost_amd.sampling import calculations
+
+    d = seeding
+    U = d["U"]
+    got = calculations.sample(U, d["input_state"], d["shots"], d["seed"], permanent_batch=oracle_batch)
+    assert Counter(map(tuple, got.tolist())) == Counter(tuple(x) for x in d["lossless"])
+    lossy = U @ np.diag([d["uniform_singular_value"]] * 5) @ U @ U.T
+    got = calculations.sample(lossy, d["input_state"], d["shots"], d["seed"], permanent_batch=oracle_batch)
+    assert Counter(map(tuple, got.tolist())) == Counter(tuple(x) for x in d["uniform_losses"])
