@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
@@ -381,12 +382,34 @@ struct BatchBuffers {
     }
 };
 
-static BatchBuffers g_batch;
+// The context's batch buffers, created on first use.
+static BatchBuffers& batch_of(Engine& E) {
+    if (!E.batch) E.batch = new BatchBuffers();
+    return *E.batch;
+}
 
-static int ensure_batch_buffers_locked(size_t nt, size_t ni, size_t nd) {
-    BatchBuffers& B = g_batch;
+// Releases a context's batch buffers and graphs (its stream is drained); the
+// context lock is held.
+void release_batch_locked(Engine& E) {
+    if (!E.batch) return;
+    BatchBuffers& B = *E.batch;
+    B.drop_graphs();
+    if (B.dbuf) (void)hipFree(B.dbuf);
+    if (B.hbuf) (void)hipHostFree(B.hbuf);
+    B.dbuf = B.hbuf = nullptr;
+    B.cap_tasks = B.cap_items = B.cap_data = 0;
+    for (int& g : B.grid) g = 0;
+    B.last_range = -1;
+    B.last_ntasks = 0;
+}
+
+// The context whose batch ran last: pqb_batch_replay replays it.
+static std::atomic<Engine*> g_last_batch{nullptr};
+
+static int ensure_batch_buffers_locked(Engine& E, size_t nt, size_t ni, size_t nd) {
+    BatchBuffers& B = batch_of(E);
     if (B.dbuf && nt <= B.cap_tasks && ni <= B.cap_items && nd <= B.cap_data) return PQB_OK;
-    (void)hipStreamSynchronize(g_eng.stream);
+    (void)hipStreamSynchronize(E.stream);
     B.drop_graphs();
     if (B.dbuf) (void)hipFree(B.dbuf);
     if (B.hbuf) (void)hipHostFree(B.hbuf);
@@ -419,8 +442,8 @@ static int ensure_batch_buffers_locked(size_t nt, size_t ni, size_t nd) {
     for (int m = 0; m < kBinomStride; ++m)
         for (int k = 0; k < kBinomStride; ++k) bt[m * kBinomStride + k] = (double)binom_u64(m, k);
     // counter + header start at zero on the device
-    e = hipMemcpyAsync(B.dbuf, B.hbuf, B.off_tasks, hipMemcpyHostToDevice, g_eng.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    e = hipMemcpyAsync(B.dbuf, B.hbuf, B.off_tasks, hipMemcpyHostToDevice, E.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
     if (e != hipSuccess) {
         release();
         return hip_fail(e, "batch init");
@@ -428,8 +451,8 @@ static int ensure_batch_buffers_locked(size_t nt, size_t ni, size_t nd) {
     return PQB_OK;
 }
 
-static RepBatchParams batch_params() {
-    BatchBuffers& B = g_batch;
+static RepBatchParams batch_params(Engine& E) {
+    BatchBuffers& B = batch_of(E);
     char* d = static_cast<char*>(B.dbuf);
     RepBatchParams p;
     p.tasks = reinterpret_cast<const RepTask*>(d + B.off_tasks);
@@ -443,29 +466,29 @@ static RepBatchParams batch_params() {
     return p;
 }
 
-static int batch_grid_locked(int range) {
-    BatchBuffers& B = g_batch;
+static int batch_grid_locked(Engine& E, int range) {
+    BatchBuffers& B = batch_of(E);
     if (B.grid[range] == 0) {
         int nb = 0;
         const void* fn = range >= kDDRange0 ? rep_dd_kernel_fn(range) : rep_batch_kernel_fn(range);
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0);
         if (e != hipSuccess || nb <= 0) nb = 1;
-        B.grid[range] = nb * g_eng.cus;
+        B.grid[range] = nb * E.cus;
     }
     return B.grid[range];
 }
 
-static int batch_graph_locked(int range, hipGraphExec_t* out) {
-    BatchBuffers& B = g_batch;
+static int batch_graph_locked(Engine& E, int range, hipGraphExec_t* out) {
+    BatchBuffers& B = batch_of(E);
     if (!B.graph[range]) {
-        const int grid = batch_grid_locked(range);
-        const RepBatchParams p = batch_params();
+        const int grid = batch_grid_locked(E, range);
+        const RepBatchParams p = batch_params(E);
         hipGraph_t g = nullptr;
-        hipError_t e = hipStreamBeginCapture(g_eng.stream, hipStreamCaptureModeThreadLocal);
+        hipError_t e = hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-        hipError_t le = range >= kDDRange0 ? launch_rep_dd(range, p, grid, g_eng.stream)
-                                           : launch_rep_batch(range, p, grid, g_eng.stream);
-        e = hipStreamEndCapture(g_eng.stream, &g);
+        hipError_t le = range >= kDDRange0 ? launch_rep_dd(range, p, grid, E.stream)
+                                           : launch_rep_batch(range, p, grid, E.stream);
+        e = hipStreamEndCapture(E.stream, &g);
         if (le != hipSuccess) return hip_fail(le, "batch launch (capture)");
         if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
         hipGraphExec_t ex = nullptr;
@@ -479,7 +502,7 @@ static int batch_graph_locked(int range, hipGraphExec_t* out) {
 }
 
 // Walk lengths set by the autotuner, indexed by photon count (0 = automatic).
-static int64_t g_walk[PQB_MAX_COLS + 1] = {0};
+static std::atomic<int64_t> g_walk[PQB_MAX_COLS + 1] = {};
 
 // Target walk length for a task of S steps when the batch holds `total`
 // steps: long walks amortise the per-chunk seed, but there must be enough
@@ -508,8 +531,8 @@ static uint64_t choose_L(long double S, long double total, int64_t lanes, bool f
 
 // Run a list of planned tasks; results (6 doubles each) into out6.  dd: the
 // double-double sweep (a chunk per quad of lanes) instead of the FP64 one.
-static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& out6, bool dd) {
-    BatchBuffers& B = g_batch;
+static int run_tasks_locked(Engine& E, std::vector<TaskPlan>& plans, std::vector<double>& out6, bool dd) {
+    BatchBuffers& B = batch_of(E);
     const size_t nt = plans.size();
     out6.assign(nt * 6, 0.0);
     // sweeps to run
@@ -543,9 +566,9 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
             }
         }
         std::vector<double> olo, ohi;
-        int st = run_tasks_locked(lo, olo, dd);
+        int st = run_tasks_locked(E, lo, olo, dd);
         if (st) return st;
-        st = run_tasks_locked(hi, ohi, dd);
+        st = run_tasks_locked(E, hi, ohi, dd);
         if (st) return st;
         for (size_t k = 0; k < ilo.size(); ++k) std::copy(&olo[6 * k], &olo[6 * k] + 6, &out6[6 * ilo[k]]);
         for (size_t k = 0; k < ihi.size(); ++k) std::copy(&ohi[6 * k], &ohi[6 * k] + 6, &out6[6 * ihi[k]]);
@@ -572,7 +595,7 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     // walker, so that the whole grid has work.
     uint64_t cap = ~(uint64_t)0;
     {
-        const uint64_t target = 4 * (uint64_t)batch_grid_locked(range) * (kBlock / kWave);
+        const uint64_t target = 4 * (uint64_t)batch_grid_locked(E, range) * (kBlock / kWave);
         uint64_t n_default = 0, chunks = 0;
         for (size_t k : live) {
             n_default += (plans[k].nchunks + per_item_of(plans[k]) - 1) / per_item_of(plans[k]);
@@ -649,7 +672,7 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
             t_off[k] = it->second;
         }
     }
-    int st = ensure_batch_buffers_locked(live.size(), items.size(), nd);
+    int st = ensure_batch_buffers_locked(E, live.size(), items.size(), nd);
     if (st) return st;
     char* h = static_cast<char*>(B.hbuf);
     char* d = static_cast<char*>(B.dbuf);
@@ -699,24 +722,25 @@ static int run_tasks_locked(std::vector<TaskPlan>& plans, std::vector<double>& o
     unsigned* O = reinterpret_cast<unsigned*>(h + B.off_order);
     std::memcpy(O, order.data(), order.size() * sizeof(unsigned));
 
-    hipError_t e = hipMemcpyAsync(d, h, B.off_data + nd, hipMemcpyHostToDevice, g_eng.stream);
+    hipError_t e = hipMemcpyAsync(d, h, B.off_data + nd, hipMemcpyHostToDevice, E.stream);
     if (e != hipSuccess) return hip_fail(e, "batch upload");
     hipGraphExec_t gx = nullptr;
-    st = batch_graph_locked(range, &gx);
+    st = batch_graph_locked(E, range, &gx);
     if (st) {
-        (void)hipStreamSynchronize(g_eng.stream);  // the upload may still read the staging buffer
+        (void)hipStreamSynchronize(E.stream);  // the upload may still read the staging buffer
         return st;
     }
-    e = hipGraphLaunch(gx, g_eng.stream);
-    if (e != hipSuccess) return drain_fail(e, "hipGraphLaunch");
+    e = hipGraphLaunch(gx, E.stream);
+    if (e != hipSuccess) return drain_fail(E, e, "hipGraphLaunch");
     std::vector<double> res(live.size() * 6);
     e = hipMemcpyAsync(res.data(), d + B.off_out, res.size() * sizeof(double), hipMemcpyDeviceToHost,
-                       g_eng.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+                       E.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
     if (e != hipSuccess) return hip_fail(e, "batch sweep");
     for (size_t k = 0; k < live.size(); ++k) std::copy(&res[6 * k], &res[6 * k] + 6, &out6[6 * live[k]]);
     B.last_range = range;
     B.last_ntasks = (int)live.size();
+    g_last_batch.store(&E);
     return PQB_OK;
 }
 
@@ -824,13 +848,14 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
         }
         return PQB_OK;
     }
-    std::lock_guard<std::mutex> lk(g_eng.mu);
-    int st = g_eng.ensure_init_locked(-1);
+    Lease L;
+    int st = L.status();
     if (st) return st;
+    Engine& E = *L;
     // chunks in flight: a lane each (FP64), a group of lanes each (double-double)
     const int dd_range = rep_dd_range_for(max_photons);
-    const int64_t lanes = dd ? (int64_t)batch_grid_locked(dd_range) * (kBlock / kWave) * rep_dd_walkers(dd_range)
-                             : (int64_t)batch_grid_locked(2) * kBlock;
+    const int64_t lanes = dd ? (int64_t)batch_grid_locked(E, dd_range) * (kBlock / kWave) * rep_dd_walkers(dd_range)
+                             : (int64_t)batch_grid_locked(E, 2) * kBlock;
     const bool large_batch = total >= 1024.0L * (long double)lanes;  // steps per lane
     auto plan_range = [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; ++i) {
@@ -842,7 +867,7 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
             // the automatic rule (a 6e6-step task at the tuned 1024-step walk left all but
             // ~6000 lanes idle: 0.32 ms against 0.10 ms).
             const int64_t tuned =
-                (!dd && large_batch && photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons] : 0;
+                (!dd && large_batch && photons >= 0 && photons <= PQB_MAX_COLS) ? g_walk[photons].load() : 0;
             const uint64_t L = L_override > 0 ? (uint64_t)L_override
                                : tuned > 0    ? (uint64_t)tuned
                                               : choose_L(steps[(size_t)i], total, lanes, !dd);
@@ -863,7 +888,7 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
             return pst[(size_t)i];
         }
     std::vector<double> out6;
-    st = run_tasks_locked(plans, out6, dd);
+    st = run_tasks_locked(E, plans, out6, dd);
     if (st) return st;
     for (int64_t i = 0; i < ntasks; ++i) {
         out[2 * i] = out6[(size_t)(6 * i)];
@@ -872,20 +897,20 @@ int batch_entry(const pqb_task* tasks, int64_t ntasks, int algo, int precision, 
     return PQB_OK;
 }
 
-int run_plain_dd_list_locked(const double* const* A, const int* R, const int* C, const int64_t* ld, int64_t n,
-                             double* out) {
+int run_plain_dd_list_locked(Engine& E, const double* const* A, const int* R, const int* C, const int64_t* ld,
+                             int64_t n, double* out) {
     std::vector<TaskPlan> plans((size_t)n);
     for (int64_t k = 0; k < n; ++k) {
         const std::vector<int64_t> rm((size_t)R[k], 1), cm((size_t)C[k], 1);
         const long double S = std::ldexp(1.0L, R[k] - 1);
         const int range = rep_dd_range_for(C[k]);  // the slots are the columns
-        const int64_t walkers = (int64_t)batch_grid_locked(range) * (kBlock / kWave) * rep_dd_walkers(range);
+        const int64_t walkers = (int64_t)batch_grid_locked(E, range) * (kBlock / kWave) * rep_dd_walkers(range);
         const int st = plan_task(A[k], R[k], C[k], ld[k], rm.data(), cm.data(), choose_L(S, S, walkers, false),
                                  plans[(size_t)k], true);
         if (st) return n > 1 ? fail(st, "matrix " + std::to_string(k) + ": " + t_err) : st;
     }
     std::vector<double> out6;
-    const int st = run_tasks_locked(plans, out6, true);
+    const int st = run_tasks_locked(E, plans, out6, true);
     if (st) return st;
     for (int64_t k = 0; k < n; ++k) {
         out[2 * k] = out6[6 * k];
@@ -894,20 +919,24 @@ int run_plain_dd_list_locked(const double* const* A, const int* R, const int* C,
     return PQB_OK;
 }
 
-int run_plain_dd_locked(const double* A, int R, int C, int64_t ld, double* out) {
-    return run_plain_dd_list_locked(&A, &R, &C, &ld, 1, out);
+int run_plain_dd_locked(Engine& E, const double* A, int R, int C, int64_t ld, double* out) {
+    return run_plain_dd_list_locked(E, &A, &R, &C, &ld, 1, out);
 }
 
 int batch_replay(int iters, double* ms) {
-    std::lock_guard<std::mutex> lk(g_eng.mu);
-    BatchBuffers& B = g_batch;
+    Engine* last = g_last_batch.load();
+    if (!last) return fail(PQB_ERR_ARG, "no batch has been run yet");
+    Lease L(*last);
+    if (L.status()) return L.status();
+    Engine& E = *L;
+    BatchBuffers& B = batch_of(E);
     if (B.last_range < 0 || !B.graph[B.last_range]) return fail(PQB_ERR_ARG, "no batch has been run yet");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = hipEventRecord(e0, g_eng.stream);
-    for (int i = 0; e == hipSuccess && i < iters; ++i) e = hipGraphLaunch(B.graph[B.last_range], g_eng.stream);
-    if (e == hipSuccess) e = hipEventRecord(e1, g_eng.stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, E.stream);
+    for (int i = 0; e == hipSuccess && i < iters; ++i) e = hipGraphLaunch(B.graph[B.last_range], E.stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, E.stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float f = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
@@ -960,8 +989,7 @@ int pqb_perm_batch_states(const double* U, int64_t rows, int64_t cols, int64_t l
 }
 
 int pqb_set_walk_lengths(const int64_t* walk, int64_t count) {
-    std::lock_guard<std::mutex> lk(g_eng.mu);
-    for (int i = 0; i <= PQB_MAX_COLS; ++i) g_walk[i] = (walk && i < count && walk[i] > 0) ? walk[i] : 0;
+    for (int i = 0; i <= PQB_MAX_COLS; ++i) g_walk[i].store((walk && i < count && walk[i] > 0) ? walk[i] : 0);
     return PQB_OK;
 }
 

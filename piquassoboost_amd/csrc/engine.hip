@@ -6,11 +6,14 @@
 // launches on the engine's stream, and error reporting (status code +
 // thread-local message; no C++ exception crosses the ABI).
 //
-// The host-pointer entry points are synchronous and serialised by one mutex
-// (the reference calculators are synchronous too: the GIL is held and TBB
-// blocks, GlynnPermanentCalculator_Wrapper.hpp:200-377).  The device-pointer
-// entry point takes the caller's stream and workspace and touches no engine
-// state besides the cached launch plans.
+// The host-pointer entry points are synchronous (as the reference calculators
+// are: the GIL is held and TBB blocks, GlynnPermanentCalculator_Wrapper.hpp:
+// 200-377) and thread safe: each call leases one of kEngineContexts execution
+// contexts (stream, device workspace, pinned staging, batch buffers and graphs)
+// for its duration, so calls from different threads run on different streams
+// at the same time.  The device-pointer entry points take the caller's stream
+// and workspace and touch no context, only the shared device state (binding,
+// occupancy of the kernels).
 
 #include <hip/hip_runtime.h>
 
@@ -19,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -41,52 +45,111 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(PQB_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-int drain_fail(hipError_t e, const char* what) {
-    if (g_eng.stream) (void)hipStreamSynchronize(g_eng.stream);
+int drain_fail(Engine& E, hipError_t e, const char* what) {
+    if (E.stream) (void)hipStreamSynchronize(E.stream);
     return hip_fail(e, what);
 }
 
-Engine g_eng;
+DeviceState g_dev;
+Engine g_ctx[kEngineContexts];
 
-// ------------------------------------------------------------------ engine --
+// ----------------------------------------------------------- device state --
 
-int Engine::ensure_init_locked(int dev) {
-    if (device >= 0 && (dev < 0 || dev == device)) {
-        hipError_t e = hipSetDevice(device);
-        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-        return PQB_OK;
-    }
+// Binding under g_dev.mu (held): device `dev` (< 0: 0) becomes the engine's.
+static int set_device_locked(int dev) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n <= 0)
         return fail(PQB_ERR_DEVICE, "no HIP device available (the permanent engine runs on the GPU only)");
     if (dev < 0) dev = 0;
     if (dev >= n) return fail(PQB_ERR_ARG, "device index out of range");
-    if (stream) {
-        (void)hipStreamSynchronize(stream);
-        (void)hipStreamDestroy(stream);
-        stream = nullptr;
-    }
-    free_buffers();
     e = hipSetDevice(dev);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, dev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
-    cus = prop.multiProcessorCount;
-    e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
-    for (int i = 0; i <= PQB_MAX_COLS; ++i) blocks_per_cu[i] = 0;
-    device = dev;
+    g_dev.cus = prop.multiProcessorCount;
+    for (int i = 0; i <= PQB_MAX_COLS; ++i) g_dev.blocks_per_cu[i] = 0;
+    g_dev.device = dev;
     return PQB_OK;
 }
 
-void Engine::free_buffers() {
+int bind_device(int dev, int* device, int* cus) {
+    {
+        std::lock_guard<std::mutex> lk(g_dev.mu);
+        const bool keep = g_dev.device >= 0 && (dev < 0 || dev == g_dev.device);
+        // the first binding: no context has a stream or buffers yet (they are
+        // created only after a binding), so nothing needs releasing
+        const int st = keep ? PQB_OK : (g_dev.device < 0 ? set_device_locked(dev) : -1);
+        if (st >= 0) {
+            if (st == PQB_OK) {
+                if (device) *device = g_dev.device;
+                if (cus) *cus = g_dev.cus;
+            }
+            return st;
+        }
+    }
+    // a change of device (pqb_init, never called from inside a context's lease):
+    // every context is locked -- in index order, before g_dev.mu, the order every
+    // caller follows -- and releases its stream and buffers first
+    std::unique_lock<std::mutex> held[kEngineContexts];
+    for (int k = 0; k < kEngineContexts; ++k) held[k] = std::unique_lock<std::mutex>(g_ctx[k].mu);
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    if (!(g_dev.device >= 0 && (dev < 0 || dev == g_dev.device))) {
+        for (int k = 0; k < kEngineContexts; ++k) g_ctx[k].release_locked();
+        const int st = set_device_locked(dev);
+        if (st) return st;
+    }
+    if (device) *device = g_dev.device;
+    if (cus) *cus = g_dev.cus;
+    return PQB_OK;
+}
+
+int64_t resident_waves(int nc) {
+    if (nc < 1 || nc > PQB_MAX_COLS) return 0;
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    if (g_dev.blocks_per_cu[nc] == 0) {
+        int nb = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, plain_kernel_fn(nc), kBlock, 0);
+        if (e != hipSuccess || nb <= 0) nb = 1;
+        g_dev.blocks_per_cu[nc] = nb;
+    }
+    return (int64_t)g_dev.blocks_per_cu[nc] * (kBlock / kWave) * g_dev.cus;
+}
+
+// ---------------------------------------------------------------- contexts --
+
+void release_batch_locked(Engine& E);  // batch.hip
+
+void Engine::release_locked() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    release_batch_locked(*this);
     if (dbuf) (void)hipFree(dbuf);
     if (hbuf) (void)hipHostFree(hbuf);
     dbuf = nullptr;
     hbuf = nullptr;
     dbytes = hbytes = 0;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    device = -1;
+}
+
+int Engine::ensure_init_locked() {
+    int dev = -1, ncu = 0;
+    int st = bind_device(-1, &dev, &ncu);
+    if (st) return st;
+    hipError_t e = hipSetDevice(dev);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if (device == dev && stream) return PQB_OK;
+    release_locked();
+    e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        stream = nullptr;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    device = dev;
+    cus = ncu;
+    return PQB_OK;
 }
 
 int Engine::ensure_buffers_locked(size_t dneed, size_t hneed) {
@@ -102,7 +165,10 @@ int Engine::ensure_buffers_locked(size_t dneed, size_t hneed) {
         // photon) reallocates a few times, not at every step
         const size_t sz = std::max<size_t>(std::max<size_t>(dneed, 2 * old_d), (size_t)1 << 20);
         hipError_t e = hipMalloc(&dbuf, sz);
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)");
+        if (e != hipSuccess) {
+            dbuf = nullptr;
+            return hip_fail(e, "hipMalloc(workspace)");
+        }
         // the grid-reduction counter at offset 0 must start at zero
         e = hipMemsetAsync(dbuf, 0, kCounterBytes, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -118,22 +184,41 @@ int Engine::ensure_buffers_locked(size_t dneed, size_t hneed) {
         }
         const size_t sz = std::max<size_t>(std::max<size_t>(hneed, 2 * old_h), (size_t)1 << 20);
         hipError_t e = hipHostMalloc(&hbuf, sz, hipHostMallocDefault);
-        if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)");
+        if (e != hipSuccess) {
+            hbuf = nullptr;
+            return hip_fail(e, "hipHostMalloc(staging)");
+        }
         hbytes = sz;
     }
     return PQB_OK;
 }
 
-int64_t Engine::resident_waves_locked(int nc) {
-    if (nc < 1 || nc > PQB_MAX_COLS) return 0;
-    if (blocks_per_cu[nc] == 0) {
-        int nb = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, plain_kernel_fn(nc), kBlock, 0);
-        if (e != hipSuccess || nb <= 0) nb = 1;
-        blocks_per_cu[nc] = nb;
-    }
-    return (int64_t)blocks_per_cu[nc] * (kBlock / kWave) * cus;
+// A thread keeps using the context it was first given when that one is free,
+// so a single-threaded caller always runs on one stream and one set of buffers.
+static int preferred_context() {
+    static std::atomic<unsigned> next{0};
+    thread_local int pref = (int)(next.fetch_add(1u) % (unsigned)kEngineContexts);
+    return pref;
 }
+
+Lease::Lease() {
+    const int pref = preferred_context();
+    for (int k = 0; k < kEngineContexts && !e_; ++k) {
+        Engine& c = g_ctx[(pref + k) % kEngineContexts];
+        std::unique_lock<std::mutex> l(c.mu, std::try_to_lock);
+        if (l.owns_lock()) {
+            lk_ = std::move(l);
+            e_ = &c;
+        }
+    }
+    if (!e_) {
+        e_ = &g_ctx[pref];
+        lk_ = std::unique_lock<std::mutex>(e_->mu);
+    }
+    st_ = e_->ensure_init_locked();
+}
+
+Lease::Lease(Engine& e) : e_(&e), lk_(e.mu) { st_ = e_->ensure_init_locked(); }
 
 // --------------------------------------------------------------- planning --
 
@@ -243,10 +328,10 @@ typedef __int128 i128;
 static double i128_to_double(i128 v) { return (double)v; }
 
 // Exact route: sum_delta (prod delta) prod_j c_j  (mod 2^128) on the GPU,
-// then divide by 2^(R-1).  Caller holds the engine lock.
-static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* result) {
+// then divide by 2^(R-1).  The caller holds the lease of E.
+static int run_exact_locked(Engine& E, const std::vector<int>& ai, int R, int C, i128* result) {
     const int T = R - 1;
-    int64_t lr = g_eng.resident_waves_locked(C) * kWave;
+    int64_t lr = E.resident_waves_locked(C) * kWave;
     const int lg = floor_log2(std::max<int64_t>(lr, kWave));
     const int H = std::min(T, std::max(0, lg));
     const int K = T - H;
@@ -260,11 +345,11 @@ static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* resu
     const size_t part_bytes = (size_t)blocks * 2 * sizeof(unsigned long long);
     const size_t out_off = part_off + ((part_bytes + 255) / 256) * 256;
     const size_t a_off = out_off + 256;
-    int st = g_eng.ensure_buffers_locked(a_off + a_bytes, a_bytes + 256);
+    int st = E.ensure_buffers_locked(a_off + a_bytes, a_bytes + 256);
     if (st) return st;
-    char* d = static_cast<char*>(g_eng.dbuf);
-    std::memcpy(g_eng.hbuf, ai.data(), a_bytes);
-    hipError_t e = hipMemcpyAsync(d + a_off, g_eng.hbuf, a_bytes, hipMemcpyHostToDevice, g_eng.stream);
+    char* d = static_cast<char*>(E.dbuf);
+    std::memcpy(E.hbuf, ai.data(), a_bytes);
+    hipError_t e = hipMemcpyAsync(d + a_off, E.hbuf, a_bytes, hipMemcpyHostToDevice, E.stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     // every running column sum is bounded by its column's abs sum: 28^13, 1400^6 and
     // (2^30)^2 are below 2^63 (integer_matrix admits column abs sums below 2^30)
@@ -284,11 +369,11 @@ static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* resu
     p.partials = reinterpret_cast<unsigned long long*>(d + part_off);
     p.counter = reinterpret_cast<unsigned*>(d);
     p.out = reinterpret_cast<unsigned long long*>(d + out_off);
-    e = launch_plain_exact(C, p, (int)blocks, block, g_eng.stream);
-    if (e != hipSuccess) return drain_fail(e, "exact sweep launch");
-    unsigned long long* hout = reinterpret_cast<unsigned long long*>(static_cast<char*>(g_eng.hbuf) + a_bytes);
-    e = hipMemcpyAsync(hout, d + out_off, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g_eng.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    e = launch_plain_exact(C, p, (int)blocks, block, E.stream);
+    if (e != hipSuccess) return drain_fail(E, e, "exact sweep launch");
+    unsigned long long* hout = reinterpret_cast<unsigned long long*>(static_cast<char*>(E.hbuf) + a_bytes);
+    e = hipMemcpyAsync(hout, d + out_off, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, E.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
     if (e != hipSuccess) return hip_fail(e, "exact sweep");
     const unsigned __int128 u = ((unsigned __int128)hout[1] << 64) | hout[0];
     const i128 sum = (i128)u;
@@ -298,18 +383,18 @@ static int run_exact_locked(const std::vector<int>& ai, int R, int C, i128* resu
     return PQB_OK;
 }
 
-// FP64 route.  Caller holds the engine lock.
-static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* out) {
-    const PlainPlan pl = make_plain_plan(R, C, g_eng.resident_waves_locked(C));
+// FP64 route.  The caller holds the lease of E.
+static int run_plain_locked(Engine& E, const double* A, int R, int C, int64_t ld, double* out) {
+    const PlainPlan pl = make_plain_plan(R, C, E.resident_waves_locked(C));
     const size_t m2_elems = (size_t)2 * R * C;
     const size_t m2_bytes = m2_elems * sizeof(double2);
     const size_t part_off = kCounterBytes;
     const size_t part_bytes = (size_t)pl.blocks * sizeof(ddc);
     const size_t out_off = part_off + ((part_bytes + 255) / 256) * 256;
     const size_t m2_off = out_off + 256;
-    int st = g_eng.ensure_buffers_locked(m2_off + m2_bytes, m2_bytes + 256);
+    int st = E.ensure_buffers_locked(m2_off + m2_bytes, m2_bytes + 256);
     if (st) return st;
-    double* h = static_cast<double*>(g_eng.hbuf);
+    double* h = static_cast<double*>(E.hbuf);
     for (int i = 0; i < R; ++i) {
         for (int j = 0; j < C; ++j) {
             const double re = A[2 * (i * ld + j)], im = A[2 * (i * ld + j) + 1];
@@ -320,8 +405,8 @@ static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* o
             h[2 * (k + (size_t)R * C) + 1] = -2.0 * im;
         }
     }
-    char* d = static_cast<char*>(g_eng.dbuf);
-    hipError_t e = hipMemcpyAsync(d + m2_off, h, m2_bytes, hipMemcpyHostToDevice, g_eng.stream);
+    char* d = static_cast<char*>(E.dbuf);
+    hipError_t e = hipMemcpyAsync(d + m2_off, h, m2_bytes, hipMemcpyHostToDevice, E.stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     PlainParams p;
     p.M2 = reinterpret_cast<const double2*>(d + m2_off);
@@ -334,11 +419,11 @@ static int run_plain_locked(const double* A, int R, int C, int64_t ld, double* o
     p.counter = reinterpret_cast<unsigned*>(d);
     p.out = reinterpret_cast<double*>(d + out_off);
     p.scale = std::ldexp(1.0, -(R - 1));
-    e = launch_plain(C, p, pl.blocks, pl.block, 1, g_eng.stream);
-    if (e != hipSuccess) return drain_fail(e, "plain sweep launch");
+    e = launch_plain(C, p, pl.blocks, pl.block, 1, E.stream);
+    if (e != hipSuccess) return drain_fail(E, e, "plain sweep launch");
     double* hout = h + 2 * m2_elems;
-    e = hipMemcpyAsync(hout, d + out_off, 6 * sizeof(double), hipMemcpyDeviceToHost, g_eng.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    e = hipMemcpyAsync(hout, d + out_off, 6 * sizeof(double), hipMemcpyDeviceToHost, E.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
     if (e != hipSuccess) return hip_fail(e, "plain sweep");
     out[0] = hout[0];
     out[1] = hout[1];
@@ -392,12 +477,13 @@ int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int alg
         return fail(PQB_ERR_EXACT, is_int ? "exact mode: the permanent may exceed the 128-bit range"
                                           : "exact mode needs a real integer-valued matrix");
 
-    std::lock_guard<std::mutex> lk(g_eng.mu);
-    int st = g_eng.ensure_init_locked(-1);
+    Lease L;
+    int st = L.status();
     if (st) return st;
+    Engine& E = *L;
     if (exact_ok) {
         i128 v = 0;
-        st = run_exact_locked(ai, R, C, &v);
+        st = run_exact_locked(E, ai, R, C, &v);
         if (st) return st;
         out[0] = i128_to_double(v);
         out[1] = 0.0;
@@ -407,8 +493,8 @@ int plain_entry(const double* A, int64_t rows, int64_t cols, int64_t ld, int alg
         }
         return PQB_OK;
     }
-    if (precision == PQB_PREC_INF) return run_plain_dd_locked(A, R, C, ld, out);
-    return run_plain_locked(A, R, C, ld, out);
+    if (precision == PQB_PREC_INF) return run_plain_dd_locked(E, A, R, C, ld, out);
+    return run_plain_locked(E, A, R, C, ld, out);
 }
 
 static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -430,7 +516,7 @@ struct ListItem {
     int R, C;
 };
 
-static int list_fp64_locked(const double* const* A, const int64_t* ld, std::vector<ListItem>& items,
+static int list_fp64_locked(Engine& E, const double* const* A, const int64_t* ld, std::vector<ListItem>& items,
                             double* out) {
     if (items.empty()) return PQB_OK;
     std::stable_sort(items.begin(), items.end(), [](const ListItem& a, const ListItem& b) {
@@ -448,7 +534,7 @@ static int list_fp64_locked(const double* const* A, const int64_t* ld, std::vect
         Group g;
         g.first = k;
         g.count = e - k;
-        g.pl = make_plain_plan(items[k].R, items[k].C, g_eng.resident_waves_locked(items[k].C));
+        g.pl = make_plain_plan(items[k].R, items[k].C, E.resident_waves_locked(items[k].C));
         groups.push_back(g);
         k = e;
     }
@@ -467,9 +553,9 @@ static int list_fp64_locked(const double* const* A, const int64_t* ld, std::vect
         m2_total += g.count * (size_t)2 * items[g.first].R * items[g.first].C * sizeof(double2);
     }
     const size_t m2_base = off;
-    int st = g_eng.ensure_buffers_locked(m2_base + m2_total, m2_total + n * 6 * sizeof(double));
+    int st = E.ensure_buffers_locked(m2_base + m2_total, m2_total + n * 6 * sizeof(double));
     if (st) return st;
-    double* h = static_cast<double*>(g_eng.hbuf);
+    double* h = static_cast<double*>(E.hbuf);
     for (const Group& g : groups) {
         const int R = items[g.first].R, C = items[g.first].C;
         const size_t rc = (size_t)R * C;
@@ -490,10 +576,10 @@ static int list_fp64_locked(const double* const* A, const int64_t* ld, std::vect
                 }
         }
     }
-    char* d = static_cast<char*>(g_eng.dbuf);
-    hipError_t e = hipMemsetAsync(d + cnt_off, 0, n * sizeof(unsigned), g_eng.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d + m2_base, h, m2_total, hipMemcpyHostToDevice, g_eng.stream);
-    if (e != hipSuccess) return drain_fail(e, "list upload");
+    char* d = static_cast<char*>(E.dbuf);
+    hipError_t e = hipMemsetAsync(d + cnt_off, 0, n * sizeof(unsigned), E.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + m2_base, h, m2_total, hipMemcpyHostToDevice, E.stream);
+    if (e != hipSuccess) return drain_fail(E, e, "list upload");
     constexpr size_t kMaxGridY = 65535;
     for (const Group& g : groups) {
         const int R = items[g.first].R, C = items[g.first].C;
@@ -511,14 +597,14 @@ static int list_fp64_locked(const double* const* A, const int64_t* ld, std::vect
             p.counter = reinterpret_cast<unsigned*>(d + cnt_off) + g.first + o;
             p.out = reinterpret_cast<double*>(d + out_off) + 6 * (g.first + o);
             p.scale = std::ldexp(1.0, -(R - 1));
-            e = launch_plain(C, p, g.pl.blocks, g.pl.block, (int)nt, g_eng.stream);
-            if (e != hipSuccess) return drain_fail(e, "list sweep launch");
+            e = launch_plain(C, p, g.pl.blocks, g.pl.block, (int)nt, E.stream);
+            if (e != hipSuccess) return drain_fail(E, e, "list sweep launch");
         }
     }
     double* hout = h + m2_total / sizeof(double);
-    e = hipMemcpyAsync(hout, d + out_off, n * 6 * sizeof(double), hipMemcpyDeviceToHost, g_eng.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
-    if (e != hipSuccess) return drain_fail(e, "list sweep");
+    e = hipMemcpyAsync(hout, d + out_off, n * 6 * sizeof(double), hipMemcpyDeviceToHost, E.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
+    if (e != hipSuccess) return drain_fail(E, e, "list sweep");
     for (size_t k = 0; k < n; ++k) {
         out[2 * items[k].idx] = hout[6 * k];
         out[2 * items[k].idx + 1] = hout[6 * k + 1];
@@ -562,12 +648,13 @@ int list_entry(const double* const* A, const int64_t* rows, const int64_t* cols,
             fp64.push_back(it);
     }
     if (exact.empty() && inf.empty() && fp64.empty()) return PQB_OK;
-    std::lock_guard<std::mutex> lk(g_eng.mu);
-    int st = g_eng.ensure_init_locked(-1);
+    Lease L;
+    int st = L.status();
     if (st) return st;
+    Engine& E = *L;
     for (auto& ex : exact) {
         i128 v = 0;
-        st = run_exact_locked(ex.second, (int)rows[ex.first], (int)cols[ex.first], &v);
+        st = run_exact_locked(E, ex.second, (int)rows[ex.first], (int)cols[ex.first], &v);
         if (st) return bad(ex.first, st, t_err);
         out[2 * ex.first] = i128_to_double(v);
         out[2 * ex.first + 1] = 0.0;
@@ -583,14 +670,15 @@ int list_entry(const double* const* A, const int64_t* rows, const int64_t* cols,
             pl.push_back(ld[it.idx]);
         }
         std::vector<double> o(2 * inf.size());
-        st = run_plain_dd_list_locked(pa.data(), pr.data(), pc.data(), pl.data(), (int64_t)inf.size(), o.data());
+        st = run_plain_dd_list_locked(E, pa.data(), pr.data(), pc.data(), pl.data(), (int64_t)inf.size(),
+                                      o.data());
         if (st) return st;
         for (size_t k = 0; k < inf.size(); ++k) {
             out[2 * inf[k].idx] = o[2 * k];
             out[2 * inf[k].idx + 1] = o[2 * k + 1];
         }
     }
-    return list_fp64_locked(A, ld, fp64, out);
+    return list_fp64_locked(E, A, ld, fp64, out);
 }
 
 // ----------------------------------------------------- device-pointer API --
@@ -617,13 +705,9 @@ int plain_device_check(int64_t count, int64_t rows, int64_t cols) {
 // Launch plan of a batch of `count` equal-size matrices: the single-matrix
 // plan for the resident waves divided among the matrices.
 int plain_device_blocks(int64_t count, int R, int C, PlainPlan* plan, int64_t* blocks_per_matrix) {
-    int64_t wr;
-    {
-        std::lock_guard<std::mutex> lk(g_eng.mu);
-        int st = g_eng.ensure_init_locked(-1);
-        if (st) return st;
-        wr = g_eng.resident_waves_locked(C);
-    }
+    const int st = bind_device(-1, nullptr, nullptr);
+    if (st) return st;
+    const int64_t wr = resident_waves(C);
     const PlainPlan pl = make_plain_plan(R, C, std::max<int64_t>(1, wr / count));
     if (pl.blocks > kMaxBlocks) return fail(PQB_ERR_SIZE, "launch plan exceeds the workspace layout");
     if (plan) *plan = pl;
@@ -696,10 +780,7 @@ int pqb_device_count(void) {
     return n;
 }
 
-int pqb_init(int device) {
-    std::lock_guard<std::mutex> lk(g_eng.mu);
-    return g_eng.ensure_init_locked(device);
-}
+int pqb_init(int device) { return bind_device(device, nullptr, nullptr); }
 
 int pqb_perm(const double* A, int64_t rows, int64_t cols, int64_t ld, int algo, int precision,
              double* out) {
@@ -781,10 +862,9 @@ int pqb_plain_plan(int64_t rows, int64_t cols, int64_t waves_resident, int64_t* 
         return fail(PQB_ERR_ARG, "no sweep for this shape");
     int64_t wr = waves_resident;
     if (wr <= 0) {
-        std::lock_guard<std::mutex> lk(g_eng.mu);
-        int st = g_eng.ensure_init_locked(-1);
+        const int st = bind_device(-1, nullptr, nullptr);
         if (st) return st;
-        wr = g_eng.resident_waves_locked((int)cols);
+        wr = resident_waves((int)cols);
     }
     const PlainPlan pl = make_plain_plan((int)rows, (int)cols, wr);
     plan[0] = pl.K;
@@ -801,13 +881,9 @@ int pqb_time_perm_device(const double* dA, int64_t rows, int64_t cols, double* d
     if (!ms || iters < 1) return fail(PQB_ERR_ARG, "bad timing arguments");
     const size_t wb = pqb_perm_device_workspace(rows, cols);
     void* w = nullptr;
-    hipStream_t st;
-    {
-        std::lock_guard<std::mutex> lk(g_eng.mu);
-        int s = g_eng.ensure_init_locked(-1);
-        if (s) return s;
-        st = g_eng.stream;
-    }
+    Lease L;  // the timed launches run on this context's stream, held for the call
+    if (L.status()) return L.status();
+    hipStream_t st = L->stream;
     hipError_t e = hipMalloc(&w, wb);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(timing workspace)");
     e = hipMemsetAsync(w, 0, kCounterBytes, st);

@@ -80,10 +80,15 @@ static size_t grad_partials_bytes(int n, int64_t count, int64_t wpt, bool minors
     return (size_t)count * (size_t)(wpt / 4) * grad_partial_elems(n, minors) * sizeof(double2);
 }
 
-// Resident waves of the adjoint kernel for this shape (engine lock held).
-static int grad_resident_waves_locked(const GradPlan& g, int64_t* waves) {
+// Per-shape LDS limit of the adjoint kernels (set once as a kernel attribute)
+// and their occupancy: shared by every caller, guarded by this mutex.
+static std::mutex g_grad_mu;
+
+// Resident waves of the adjoint kernel for this shape on the bound device.
+static int grad_resident_waves(const GradPlan& g, int device, int cus, int64_t* waves) {
     const void* fn = grad_kernel_fn(g.M, g.K, g.minors);
     if (!fn) return fail(PQB_ERR_SIZE, "no adjoint kernel for this size");
+    std::lock_guard<std::mutex> lk(g_grad_mu);
     // dynamic LDS beyond the default: up to the CU's LDS minus the kernel's static LDS
     static size_t dyn_limit[11][4][2] = {};
     size_t& limit = dyn_limit[g.M][g.K][g.minors];
@@ -92,7 +97,7 @@ static int grad_resident_waves_locked(const GradPlan& g, int64_t* waves) {
         hipError_t e = hipFuncGetAttributes(&fa, fn);
         if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes(adjoint kernel)");
         hipDeviceProp_t prop;
-        e = hipGetDeviceProperties(&prop, g_eng.device);
+        e = hipGetDeviceProperties(&prop, device);
         if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
         const size_t cu_lds = prop.maxSharedMemoryPerMultiProcessor;
         if (cu_lds <= fa.sharedSizeBytes) return fail(PQB_ERR_DEVICE, "no LDS left for the adjoint kernel");
@@ -106,16 +111,17 @@ static int grad_resident_waves_locked(const GradPlan& g, int64_t* waves) {
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, g.lds);
     if (e != hipSuccess) return hip_fail(e, "occupancy query (adjoint kernel)");
     if (nb <= 0) return fail(PQB_ERR_SIZE, "adjoint kernel does not fit on a compute unit");
-    *waves = (int64_t)nb * 4 * g_eng.cus;
+    *waves = (int64_t)nb * 4 * cus;
     return PQB_OK;
 }
 
-static int grad_plan_locked(int n, int64_t count, GradPlan* out, bool minors = false) {
-    int st = g_eng.ensure_init_locked(-1);
+static int grad_plan(int n, int64_t count, GradPlan* out, bool minors = false) {
+    int device = -1, cus = 0;
+    int st = bind_device(-1, &device, &cus);
     if (st) return st;
     const GradPlan shape = grad_shape(n, minors);
     int64_t wr = 0;
-    st = grad_resident_waves_locked(shape, &wr);
+    st = grad_resident_waves(shape, device, cus, &wr);
     if (st) return st;
     *out = make_grad_plan(n, count, wr, minors);
     return PQB_OK;
@@ -181,9 +187,12 @@ static int grad_host_entry(const double* A, int64_t count, int64_t n, int64_t ld
     const int nn = (int)n;
     const size_t mat = (size_t)nn * nn;
 
-    std::lock_guard<std::mutex> lk(g_eng.mu);
+    Lease L;
+    st = L.status();
+    if (st) return st;
+    Engine& E = *L;
     GradPlan g;
-    st = grad_plan_locked(nn, count, &g);
+    st = grad_plan(nn, count, &g);
     if (st) return st;
     const size_t a_bytes = (size_t)count * mat * sizeof(double2);
     const size_t out_bytes = (size_t)count * (mat + 1) * sizeof(double2);
@@ -191,27 +200,27 @@ static int grad_host_entry(const double* A, int64_t count, int64_t n, int64_t ld
     const size_t a_off = kCounterBytes;  // offset 0 holds the plain sweep's grid counter
     const size_t out_off = a_off + ((a_bytes + 255) / 256) * 256;
     const size_t part_off = out_off + ((out_bytes + 255) / 256) * 256;
-    st = g_eng.ensure_buffers_locked(part_off + part_bytes, std::max(a_bytes, out_bytes));
+    st = E.ensure_buffers_locked(part_off + part_bytes, std::max(a_bytes, out_bytes));
     if (st) return st;
-    double* h = static_cast<double*>(g_eng.hbuf);
+    double* h = static_cast<double*>(E.hbuf);
     for (int64_t b = 0; b < count; ++b)
         for (int i = 0; i < nn; ++i)
             std::memcpy(h + 2 * ((size_t)b * mat + (size_t)i * nn), A + 2 * (b * stride + i * ld),
                         (size_t)nn * 2 * sizeof(double));
-    char* d = static_cast<char*>(g_eng.dbuf);
-    hipError_t e = hipMemcpyAsync(d + a_off, h, a_bytes, hipMemcpyHostToDevice, g_eng.stream);
+    char* d = static_cast<char*>(E.dbuf);
+    hipError_t e = hipMemcpyAsync(d + a_off, h, a_bytes, hipMemcpyHostToDevice, E.stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     double2* dperm = reinterpret_cast<double2*>(d + out_off);
     double2* dgrad = dperm + count;
     st = grad_launch(g, reinterpret_cast<const double2*>(d + a_off), nn, count, dperm, dgrad, d + part_off,
-                     g_eng.stream);
+                     E.stream);
     if (st) {
-        (void)hipStreamSynchronize(g_eng.stream);  // the upload may still read the staging buffer
+        (void)hipStreamSynchronize(E.stream);  // the upload may still read the staging buffer
         return st;
     }
     // the staging buffer is reused for the results once the upload has been consumed
-    e = hipMemcpyAsync(h, d + out_off, out_bytes, hipMemcpyDeviceToHost, g_eng.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    e = hipMemcpyAsync(h, d + out_off, out_bytes, hipMemcpyDeviceToHost, E.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
     if (e != hipSuccess) return hip_fail(e, "adjoint sweep");
     std::memcpy(perm_out, h, (size_t)count * 2 * sizeof(double));
     std::memcpy(grad_out, h + 2 * count, (size_t)count * mat * 2 * sizeof(double));
@@ -239,9 +248,12 @@ static int minors_host_entry(const double* A, int64_t count, int64_t rows, int64
     if (count > 1 && stride < rows * ld) return fail(PQB_ERR_ARG, "matrix stride smaller than rows * ld");
     const int R = (int)rows, C = (int)cols;
 
-    std::lock_guard<std::mutex> lk(g_eng.mu);
+    Lease L;
+    int st = L.status();
+    if (st) return st;
+    Engine& E = *L;
     GradPlan g;
-    int st = grad_plan_locked(R, count, &g, true);
+    st = grad_plan(R, count, &g, true);
     if (st) return st;
     const size_t mat = (size_t)R * C;
     const size_t a_bytes = (size_t)count * mat * sizeof(double2);
@@ -250,24 +262,24 @@ static int minors_host_entry(const double* A, int64_t count, int64_t rows, int64
     const size_t a_off = kCounterBytes;  // offset 0 holds the plain sweep's grid counter
     const size_t out_off = a_off + ((a_bytes + 255) / 256) * 256;
     const size_t part_off = out_off + ((out_bytes + 255) / 256) * 256;
-    st = g_eng.ensure_buffers_locked(part_off + part_bytes, std::max(a_bytes, out_bytes));
+    st = E.ensure_buffers_locked(part_off + part_bytes, std::max(a_bytes, out_bytes));
     if (st) return st;
-    double* h = static_cast<double*>(g_eng.hbuf);
+    double* h = static_cast<double*>(E.hbuf);
     for (int64_t b = 0; b < count; ++b)
         for (int i = 0; i < R; ++i)
             std::memcpy(h + 2 * ((size_t)b * mat + (size_t)i * C), A + 2 * (b * stride + i * ld),
                         (size_t)C * 2 * sizeof(double));
-    char* d = static_cast<char*>(g_eng.dbuf);
-    hipError_t e = hipMemcpyAsync(d + a_off, h, a_bytes, hipMemcpyHostToDevice, g_eng.stream);
+    char* d = static_cast<char*>(E.dbuf);
+    hipError_t e = hipMemcpyAsync(d + a_off, h, a_bytes, hipMemcpyHostToDevice, E.stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     st = grad_launch(g, reinterpret_cast<const double2*>(d + a_off), R, count, nullptr,
-                     reinterpret_cast<double2*>(d + out_off), d + part_off, g_eng.stream);
+                     reinterpret_cast<double2*>(d + out_off), d + part_off, E.stream);
     if (st) {
-        (void)hipStreamSynchronize(g_eng.stream);  // the upload may still read the staging buffer
+        (void)hipStreamSynchronize(E.stream);  // the upload may still read the staging buffer
         return st;
     }
-    e = hipMemcpyAsync(h, d + out_off, out_bytes, hipMemcpyDeviceToHost, g_eng.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g_eng.stream);
+    e = hipMemcpyAsync(h, d + out_off, out_bytes, hipMemcpyDeviceToHost, E.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
     if (e != hipSuccess) return hip_fail(e, "column-minors sweep");
     std::memcpy(out, h, out_bytes);
     return PQB_OK;
@@ -300,9 +312,8 @@ int pqb_perm_grad_workspace(int64_t count, int64_t n, size_t* bytes) {
     if (!bytes) return fail(PQB_ERR_ARG, "bytes pointer is NULL");
     int st = grad_check(count, n);
     if (st) return st;
-    std::lock_guard<std::mutex> lk(g_eng.mu);
     GradPlan g;
-    st = grad_plan_locked((int)n, count, &g);
+    st = grad_plan((int)n, count, &g);
     if (st) return st;
     *bytes = grad_partials_bytes((int)n, count, g.wpt);
     return PQB_OK;
@@ -314,11 +325,8 @@ int pqb_perm_grad_device(const double* dA, int64_t count, int64_t n, double* d_p
     int st = grad_check(count, n);
     if (st) return st;
     GradPlan g;
-    {
-        std::lock_guard<std::mutex> lk(g_eng.mu);
-        st = grad_plan_locked((int)n, count, &g);
-        if (st) return st;
-    }
+    st = grad_plan((int)n, count, &g);
+    if (st) return st;
     if (work_bytes < grad_partials_bytes((int)n, count, g.wpt)) return fail(PQB_ERR_ARG, "workspace too small");
     return grad_launch(g, reinterpret_cast<const double2*>(dA), (int)n, count, reinterpret_cast<double2*>(d_perm),
                        reinterpret_cast<double2*>(d_grad), d_work, static_cast<hipStream_t>(stream));
@@ -332,8 +340,7 @@ int pqb_grad_plan(int64_t count, int64_t n, int64_t waves_resident, int64_t* pla
     if (waves_resident > 0) {
         g = make_grad_plan((int)n, count, waves_resident);
     } else {
-        std::lock_guard<std::mutex> lk(g_eng.mu);
-        st = grad_plan_locked((int)n, count, &g);
+        st = grad_plan((int)n, count, &g);
         if (st) return st;
     }
     plan[0] = g.M;

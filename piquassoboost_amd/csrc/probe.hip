@@ -39,25 +39,26 @@ static int probe_entry(int waves_per_simd, int64_t iters, double* ms, double* tf
     if (waves_per_simd < 1 || waves_per_simd > 8) return fail(PQB_ERR_ARG, "waves_per_simd must be 1..8");
     if (iters < 1 || iters > ((int64_t)1 << 30)) return fail(PQB_ERR_ARG, "iters must be 1..2^30");
     if (!ms || !tflops) return fail(PQB_ERR_ARG, "NULL output pointer");
-    std::lock_guard<std::mutex> lk(g_eng.mu);
-    int st = g_eng.ensure_init_locked(-1);
+    Lease L;
+    int st = L.status();
     if (st) return st;
-    const int blocks = g_eng.cus * waves_per_simd;  // 4 waves per block: one per SIMD
+    Engine& E = *L;
+    const int blocks = E.cus * waves_per_simd;  // 4 waves per block: one per SIMD
     const size_t out_bytes = (size_t)blocks * kBlock * sizeof(double);
-    st = g_eng.ensure_buffers_locked(kCounterBytes + out_bytes, 0);
+    st = E.ensure_buffers_locked(kCounterBytes + out_bytes, 0);
     if (st) return st;
-    double* dout = reinterpret_cast<double*>(static_cast<char*>(g_eng.dbuf) + kCounterBytes);
+    double* dout = reinterpret_cast<double*>(static_cast<char*>(E.dbuf) + kCounterBytes);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     // one untimed launch (code load, clock ramp), then the timed one
     for (int rep = 0; e == hipSuccess && rep < 2; ++rep) {
-        if (rep == 1) e = hipEventRecord(e0, g_eng.stream);
+        if (rep == 1) e = hipEventRecord(e0, E.stream);
         if (e != hipSuccess) break;
-        hipLaunchKernelGGL(fp64_fma_probe_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, g_eng.stream,
+        hipLaunchKernelGGL(fp64_fma_probe_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, E.stream,
                            1.0 - 1e-9, 1e-9, iters, dout);
         e = hipGetLastError();
-        if (e == hipSuccess && rep == 1) e = hipEventRecord(e1, g_eng.stream);
+        if (e == hipSuccess && rep == 1) e = hipEventRecord(e1, E.stream);
     }
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float f = 0.f;

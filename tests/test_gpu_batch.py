@@ -158,9 +158,10 @@ def test_batched_front_end(uneven_batch4096):
 def test_batches_from_concurrent_host_threads(uneven_batch4096):
     """Batch calls from 4 Python threads at once (ctypes releases the GIL): their
     host phases contend for the worker pool (a phase that finds it busy starts
-    its own threads) and their sweeps take the engine lock in turn.  Every call
-    returns the values of the same call made alone, bit for bit (the task list
-    and the state-array form of one half of the batch plan identically)."""
+    its own threads) and their sweeps run on separate engine contexts (own
+    stream, buffers and captured graphs) at the same time.  Every call returns
+    the values of the same call made alone, bit for bit (the task list and the
+    state-array form of one half of the batch plan identically)."""
     from concurrent.futures import ThreadPoolExecutor
 
     U, pairs, tasks, res = uneven_batch4096

@@ -390,23 +390,39 @@ def test_repeated_list_batched():
             assert rel_err(res[i][j], oracle.bbfg_repeated(U, ins[i], o, "dd")) < 1e-12
 
 
-def test_concurrent_host_calls_are_serialised_and_deterministic():
-    """The host entry points hold one engine lock (ctypes releases the GIL around them):
-    calls from 8 Python threads give bit-identical results to sequential calls."""
+def test_concurrent_host_calls_are_deterministic():
+    """Host calls from 8 Python threads (ctypes releases the GIL around them) run on up to
+    four engine contexts -- a stream, workspaces and batch graphs each -- at once; every
+    entry point gives bit-identical results to the same calls made one after another."""
     from concurrent.futures import ThreadPoolExecutor
 
     rng = np.random.default_rng(99)
-    mats = [random_complex(rng, 14, 14) for _ in range(12)]
+    mats = [random_complex(rng, 14, 14) for _ in range(48)]
+    U = random_complex(rng, 12, 12, scale=0.3)
+    ins = np.array([[1, 1, 0, 1, 0, 1, 0, 0, 1, 0, 1, 0], [2, 0, 0, 1, 0, 1, 0, 0, 0, 0, 1, 0]])
+    outs = np.array([[0, 1, 1, 1, 0, 0, 1, 0, 0, 1, 0, 0], [1, 0, 0, 0, 2, 0, 0, 1, 0, 0, 1, 0]])
 
     def work(k):
         A = mats[k]
-        if k % 3 == 0:
+        kind = k % 8
+        if kind == 0:
             return C.permanent(A)
-        if k % 3 == 1:
+        if kind == 1:
             return C.permanent_repeated(A[:5, :5], np.array([1, 2, 1, 1, 2]), np.array([2, 1, 1, 2, 1]))
-        return complex(C.permanent_grad(A[:9, :9])[1][2, 3])
+        if kind == 2:
+            return complex(C.permanent_grad(A[:9, :9])[1][2, 3])
+        if kind == 3:
+            return tuple(C.permanent_list([A[:6, :6], A[:7, :8], A], C.ALGO_BBFG, C.PREC_FP64))
+        if kind == 4:
+            return tuple(C.permanent_batch_states(U, ins, outs).tolist())
+        if kind == 5:
+            return C.permanent(A[:10, :10], C.ALGO_BBFG, C.PREC_INF)
+        if kind == 6:
+            return C.permanent_exact(np.round(A[:11, :11].real * 4))[0]
+        return tuple(C.permanent_minors(A[:8, :9]).tolist())
 
     seq = [work(k) for k in range(len(mats))]
-    with ThreadPoolExecutor(8) as ex:
-        par = list(ex.map(work, range(len(mats))))
-    assert par == seq
+    for _ in range(2):
+        with ThreadPoolExecutor(8) as ex:
+            par = list(ex.map(work, range(len(mats))))
+        assert par == seq
