@@ -432,7 +432,7 @@ def main() -> int:
     ap.add_argument("--no-inf", action="store_true", help="skip the double-double (PQB_PREC_INF) leg")
     ap.add_argument("--no-exact", action="store_true", help="skip the exact integer (0/1 matrices) leg")
     ap.add_argument("--fma-probe", action="store_true",
-                    help="instead of the benchmark, run the FP64 FMA issue-rate probe at 1..4 waves per SIMD "
+                    help="instead of the benchmark, run the FP64 FMA issue-rate probe at 1, 2, 3, 4 and 8 waves per SIMD "
                          "(the FP64 VALU ceiling of the device; profile it for the clock)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
@@ -454,7 +454,7 @@ def main() -> int:
         from piquassoboost_amd import _capi as C
 
         res = {}
-        for w in (1, 2, 3, 4):
+        for w in (1, 2, 3, 4, 8):
             ms, tf = C.probe_fp64_fma(w)
             res[f"waves_per_simd_{w}"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP64_PEAK_TFLOPS}
             print(f"fp64 fma probe, {w} waves/SIMD: {ms:.2f} ms, {tf:.1f} TFLOP/s", file=sys.stderr, flush=True)

@@ -226,6 +226,21 @@ size_t pqb_perm_device_workspace(int64_t rows, int64_t cols);
 int pqb_perm_device(const double* dA, int64_t rows, int64_t cols, double* d_out,
                     void* d_work, size_t work_bytes, void* stream);
 
+/* pqb_perm_device without a workspace argument and without any allocation:
+ * the workspace (grid counter, per-block partials, +-2A table) is a static
+ * device array of the library, pqb_perm_device_noalloc_scratch_bytes() bytes
+ * per GPU (about 180 KB, held until the process ends), sized for the largest
+ * matrix the kernels take (41 x 40).  Same plan and kernels as
+ * pqb_perm_device, so the results are bit-identical to it.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+ * two threads) race and must be ordered by the caller.  Matrices beyond
+ * PQB_MAX_COLS columns fail with PQB_ERR_SIZE.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops); the first call on a device looks up the
+ * array's address, later calls only launch. */
+size_t pqb_perm_device_noalloc_scratch_bytes(void);
+int pqb_perm_device_noalloc(const double* dA, int64_t rows, int64_t cols, double* d_out, void* stream);
+
 /* Batch of `count` equal-size plain permanents on device pointers: dA packed
  * count x rows x cols complex128; d_out 6 doubles per matrix (as above);
  * d_work at least pqb_perm_batch_device_workspace bytes, its first
@@ -303,7 +318,7 @@ int pqb_time_perm_device(const double* dA, int64_t rows, int64_t cols, double* d
                          int iters, double* ms);
 
 /* FP64 issue-rate probe (measurement only): `waves_per_simd` waves on every
- * SIMD each run `iters` x 8 independent v_fma_f64 chains; reports the timed
+ * SIMD each run `iters` x 16 independent v_fma_f64 chains; reports the timed
  * launch's milliseconds (HIP events on the engine stream) and its FP64
  * TFLOP/s (2 flops per FMA).  The ceiling any FP64 VALU kernel can reach on
  * this device at that occupancy.  No reference counterpart. */

@@ -79,6 +79,9 @@ SIGNATURES = {
     "pqb_perm_batch_device_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                        ctypes.POINTER(ctypes.c_size_t)]),
     "pqb_probe_fp64_fma": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _dp, _dp]),
+    "pqb_perm_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
+    "pqb_perm_device_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "pqb_perm_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
 }
@@ -390,6 +393,13 @@ def permanent_device(dA_ptr: int, rows: int, cols: int, d_out_ptr: int, d_work_p
     _check(lib().pqb_perm_device(dA_ptr, rows, cols, d_out_ptr, d_work_ptr, work_bytes, stream or None))
 
 
+def permanent_device_noalloc(dA_ptr: int, rows: int, cols: int, d_out_ptr: int, stream: int = 0) -> None:
+    """Asynchronous plain permanent on device pointers with the library's static device
+    scratch buffer as workspace (see pqb_perm_device_noalloc): no allocation, no workspace
+    argument; calls that overlap on one GPU race."""
+    _check(lib().pqb_perm_device_noalloc(dA_ptr, rows, cols, d_out_ptr, stream or None))
+
+
 def time_permanent_device(dA_ptr: int, rows: int, cols: int, d_out_ptr: int, iters: int) -> float:
     ms = ctypes.c_double(0.0)
     _check(lib().pqb_time_perm_device(dA_ptr, rows, cols, d_out_ptr, iters, ctypes.byref(ms)))
@@ -479,7 +489,7 @@ def permanent_minors_batch(A) -> np.ndarray:
     return out
 
 
-def probe_fp64_fma(waves_per_simd: int, iters: int = 1 << 20) -> tuple[float, float]:
+def probe_fp64_fma(waves_per_simd: int, iters: int = 1 << 19) -> tuple[float, float]:
     """(ms, TFLOP/s) of the FP64 FMA issue-rate probe at that occupancy."""
     ms = ctypes.c_double(0.0)
     tf = ctypes.c_double(0.0)

@@ -1,6 +1,6 @@
 // FP64 issue-rate probe (measurement only): pqb_probe_fp64_fma.
 //
-// Every lane runs 8 independent chains of v_fma_f64 (no loads in the loop),
+// Every lane runs 16 independent chains of v_fma_f64 (no loads in the loop),
 // so the only limits are the FP64 pipe's issue rate and the clock.  Launched
 // as exactly `waves_per_simd` waves on every SIMD of the device (blocks of 4
 // waves, `waves_per_simd` blocks per CU), timed with HIP events on the engine
@@ -18,21 +18,21 @@
 
 namespace pqb {
 
+constexpr int kProbeChains = 16;
+
 __global__ __launch_bounds__(256) void fp64_fma_probe_kernel(double a, double b, int64_t iters, double* out) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double x0 = 1.0 + 1e-3 * (double)(threadIdx.x & 7), x1 = x0 + 1e-4, x2 = x0 + 2e-4, x3 = x0 + 3e-4;
-    double x4 = x0 + 4e-4, x5 = x0 + 5e-4, x6 = x0 + 6e-4, x7 = x0 + 7e-4;
+    double x[kProbeChains];
+#pragma unroll
+    for (int c = 0; c < kProbeChains; ++c) x[c] = 1.0 + 1e-3 * (double)(threadIdx.x & 7) + 1e-4 * c;
     for (int64_t i = 0; i < iters; ++i) {
-        x0 = fma(x0, a, b);
-        x1 = fma(x1, a, b);
-        x2 = fma(x2, a, b);
-        x3 = fma(x3, a, b);
-        x4 = fma(x4, a, b);
-        x5 = fma(x5, a, b);
-        x6 = fma(x6, a, b);
-        x7 = fma(x7, a, b);
+#pragma unroll
+        for (int c = 0; c < kProbeChains; ++c) x[c] = fma(x[c], a, b);
     }
-    out[g] = ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7));
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < kProbeChains; ++c) s += x[c];
+    out[g] = s;
 }
 
 static int probe_entry(int waves_per_simd, int64_t iters, double* ms, double* tflops) {
@@ -67,7 +67,7 @@ static int probe_entry(int waves_per_simd, int64_t iters, double* ms, double* tf
     if (e1) (void)hipEventDestroy(e1);
     if (e != hipSuccess) return hip_fail(e, "fp64 probe");
     *ms = f;
-    const double flops = (double)blocks * kBlock * (double)iters * 8.0 * 2.0;
+    const double flops = (double)blocks * kBlock * (double)iters * kProbeChains * 2.0;
     *tflops = flops / ((double)f * 1e-3) / 1e12;
     return PQB_OK;
 }
