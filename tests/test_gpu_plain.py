@@ -400,7 +400,7 @@ def test_concurrent_host_calls_are_deterministic():
     mats = [random_complex(rng, 14, 14) for _ in range(48)]
     U = random_complex(rng, 12, 12, scale=0.3)
     ins = np.array([[1, 1, 0, 1, 0, 1, 0, 0, 1, 0, 1, 0], [2, 0, 0, 1, 0, 1, 0, 0, 0, 0, 1, 0]])
-    outs = np.array([[0, 1, 1, 1, 0, 0, 1, 0, 0, 1, 0, 0], [1, 0, 0, 0, 2, 0, 0, 1, 0, 0, 1, 0]])
+    outs = np.array([[0, 1, 1, 1, 0, 0, 1, 0, 0, 1, 1, 0], [1, 0, 0, 0, 2, 0, 0, 1, 0, 0, 1, 0]])
 
     def work(k):
         A = mats[k]
