@@ -221,21 +221,27 @@ void rep_batch_kernel(RepBatchParams p) {
     }
 }
 
-// One wave per task (grid-stride over tasks): sum the task's item partials
-// (items of a task are contiguous) in double-double, scale, write; block 0
-// resets the item counter for the next launch.
-__global__ __launch_bounds__(64) void rep_reduce_kernel(RepBatchParams p) {
-    const int lane = threadIdx.x;
+// One block of 256 threads per task (grid-stride over tasks): sum the task's
+// item partials (items of a task are contiguous) in double-double in a fixed
+// order -- thread t takes items t, t+256, ... with several loads in flight, then
+// the wave butterflies and the four waves in order -- scale, write; block 0
+// resets the item counter for the next launch.  (A single 30-photon task has
+// ~2000 items, the double-double n = 30 sweep ~16000: one wave per task took
+// 11.7 us and 80 us of dependent loads for them.)
+__global__ __launch_bounds__(kReduceThreads) void rep_reduce_kernel(RepBatchParams p) {
+    __shared__ ddc red[kReduceThreads / kWave];
     const unsigned ntasks = to_const(p.counts)[1];
     for (unsigned task = blockIdx.x; task < ntasks; task += gridDim.x) {
         const RepTask& T = p.tasks[task];
         ddc s = {0.0, 0.0, 0.0, 0.0};
-        for (uint32_t i = T.item0 + lane; i < T.item0 + T.nitems; i += kWave) {
+        const uint32_t end = T.item0 + T.nitems;
+#pragma unroll 4
+        for (uint32_t i = T.item0 + threadIdx.x; i < end; i += kReduceThreads) {
             const ddc q = p.partials[i];
             ddc_add(s, q);
         }
-        s = wave_reduce(s);
-        if (lane == 0) {
+        s = block_reduce(s, red);
+        if (threadIdx.x == 0) {
             s = ddc_scale(s, T.scale);
             double* o = p.out + 6 * (size_t)T.out_index;
             o[0] = s.hr + s.lr;
@@ -245,6 +251,7 @@ __global__ __launch_bounds__(64) void rep_reduce_kernel(RepBatchParams p) {
             o[4] = s.hi;
             o[5] = s.li;
         }
+        __syncthreads();  // `red` is reused by the next task
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *p.head = 0u;
 }
@@ -285,7 +292,7 @@ hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks,
 }
 
 hipError_t launch_rep_reduce(const RepBatchParams& p, hipStream_t st) {
-    hipLaunchKernelGGL(rep_reduce_kernel, dim3(kReduceBlocks), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(rep_reduce_kernel, dim3(kReduceBlocks), dim3(kReduceThreads), 0, st, p);
     return hipGetLastError();
 }
 

@@ -86,7 +86,8 @@ struct RepBatchParams {
                             // kernels, so a captured graph serves every batch that fits)
 };
 
-constexpr int kReduceBlocks = 512;  // fixed grid of the per-task reduction (one wave per block)
+constexpr int kReduceBlocks = 512;    // fixed grid of the per-task reduction
+constexpr int kReduceThreads = 256;  // one block of four waves per task
 
 // Kernel instantiations cover slot ranges; rep_range_for() picks the narrowest
 // one for a batch (-1: the batch needs splitting at 32 slots).
