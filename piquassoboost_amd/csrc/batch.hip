@@ -352,8 +352,7 @@ struct BatchBuffers {
     void* dbuf = nullptr;
     void* hbuf = nullptr;
     size_t cap_tasks = 0, cap_items = 0, cap_data = 0;
-    size_t off_binom = 0, off_tasks = 0, off_items = 0, off_order = 0, off_done = 0, off_data = 0, off_part = 0,
-           off_out = 0,
+    size_t off_binom = 0, off_tasks = 0, off_items = 0, off_order = 0, off_data = 0, off_part = 0, off_out = 0,
            total = 0;
     hipGraphExec_t graph[kRanges] = {};  // FP64 slot ranges, then the double-double ones
     int grid[kRanges] = {};
@@ -369,8 +368,7 @@ struct BatchBuffers {
         off_tasks = align256(off_binom + sizeof(double) * kBinomStride * kBinomStride);
         off_items = align256(off_tasks + nt * sizeof(RepTask));
         off_order = align256(off_items + ni * sizeof(RepItem));
-        off_done = align256(off_order + ni * sizeof(unsigned));
-        off_data = align256(off_done + nt * sizeof(unsigned));
+        off_data = align256(off_order + ni * sizeof(unsigned));
         off_part = align256(off_data + nd);
         off_out = align256(off_part + ni * sizeof(ddc));
         total = align256(off_out + nt * 6 * sizeof(double));
@@ -678,13 +676,11 @@ static int run_tasks_locked(Engine& E, std::vector<TaskPlan>& plans, std::vector
     if (st) return st;
     char* h = static_cast<char*>(B.hbuf);
     char* d = static_cast<char*>(B.dbuf);
-    // header: item counter (0), counts, wave counter (128); per-task item counts
-    // start at zero (every upload rewrites them; the kernel leaves them zero)
+    // header: head counter (0), counts
     std::memset(h, 0, 256);
     unsigned* counts = reinterpret_cast<unsigned*>(h + 64);
     counts[0] = (unsigned)items.size();
     counts[1] = (unsigned)live.size();
-    std::memset(h + B.off_done, 0, live.size() * sizeof(unsigned));
     RepTask* T = reinterpret_cast<RepTask*>(h + B.off_tasks);
     // descriptors and per-task data into the pinned staging buffer (disjoint
     // regions per task; ~20 MB for 4096 tasks of up to 26 photons)
@@ -704,8 +700,6 @@ static int run_tasks_locked(Engine& E, std::vector<TaskPlan>& plans, std::vector
         t.item0 = task_items[k].first;
         t.nitems = task_items[k].second;
         t.out_index = (uint32_t)k;
-        t.out = reinterpret_cast<double*>(d + B.off_out) + 6 * (size_t)k;
-        t.done = reinterpret_cast<unsigned*>(d + B.off_done) + k;
         t.nchunks = (uint32_t)P.nchunks;
         t.scale = P.scale;
         for (int hd = 0; hd < P.nhigh; ++hd) {

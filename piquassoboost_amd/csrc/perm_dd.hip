@@ -42,7 +42,6 @@
 // product and sum is rounded exactly as written, and a product fused into the
 // following TwoSum silently drops the low half.
 
-#include "pqb_batch_device.h"
 #include "pqb_device.h"
 #include "pqb_kernels.h"
 
@@ -282,12 +281,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHI <= 8 ? 
         if (q != 0) acc = ddc{0.0, 0.0, 0.0, 0.0};
         const ddc v = wave_reduce(acc);
         if (lane == 0) p.partials[idx] = v;
-        rep_finish_item(p, idx, lane);
         // every lane executes the atomic, only lane 0 adds (see rep_batch_kernel)
         const unsigned got = atomicAdd(p.head, lane == 0 ? 1u : 0u);
         idx0 = nwaves + __builtin_amdgcn_readlane(got, 0);
     }
-    rep_finish_wave(p, lane);
 }
 
 // perm(A) = sum_j a_0j G_0j (Laplace expansion along row 0 of the finished
@@ -357,7 +354,9 @@ hipError_t launch_rep_dd(int range, const RepBatchParams& p, int grid_blocks, hi
         case 3: hipLaunchKernelGGL((rep_dd_kernel<4, 10>), dim3(grid_blocks), dim3(256), 0, st, p); break;
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_rep_reduce(p, st);
 }
 
 }  // namespace pqb

@@ -193,7 +193,9 @@ void plain_sweep_kernel(PlainParams p) {
         for (int64_t g = g0;;) {
             // this group's part of the run: t in [t0, t_stop)
             const int64_t gbase = g << K;
-            const unsigned t_stop = (tau1 - gbase >= (int64_t)W) ? W : (unsigned)(tau1 - gbase);
+            // wave-uniform; taken through readfirstlane so that the loop compare is scalar
+            const unsigned t_stop = (unsigned)__builtin_amdgcn_readfirstlane(
+                (int)((tau1 - gbase >= (int64_t)W) ? W : (unsigned)(tau1 - gbase)));
             double pr = 0.0, pi = 0.0, mr = 0.0, mi = 0.0;
             // t = t0: even Gray index
             add_term<NC>(cr, ci, pr, pi);

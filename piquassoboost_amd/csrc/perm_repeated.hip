@@ -30,15 +30,13 @@
 // * Work is a list of items (task, first chunk, chunk count); each wave of a
 //   persistent grid pulls items from an atomic counter, each lane takes every
 //   64th chunk of the item, and the wave writes one double-double partial per
-//   item.  The wave that finishes a task's last item sums the task's partials
-//   (double-double, fixed order) and scales by 2^-(N-1); the last wave to leave
-//   resets the counters, so the one kernel can be captured once in a hipGraph
-//   and replayed for every batch.
+//   item.  A second kernel sums each task's partials (double-double, fixed
+//   order), scales by 2^-(N-1), and resets the item counter, so the pair can be
+//   captured once in a hipGraph and replayed for every batch.
 // * The kernel is compiled for slot counts 1..kMaxSlots and switches on the
 //   item's slot count (wave-uniform), so one launch serves a batch of uneven
 //   sizes.
 
-#include "pqb_batch_device.h"
 #include "pqb_device.h"
 #include "pqb_kernels.h"
 
@@ -213,7 +211,6 @@ void rep_batch_kernel(RepBatchParams p) {
         rep_dispatch<LO, HI>(T->NS, T, p.binom, it.chunk0, it.count, lane, sr, si);
         const ddc v = wave_reduce(ddc_from(sr, si));
         if (lane == 0) p.partials[idx] = v;
-        rep_finish_item(p, idx, lane);
         // Every lane executes the atomic (uniform control flow); only lane 0
         // adds 1, and lane 0's old value is the wave's next item.  A
         // lane-0-only branch around the atomic must not be used here: the
@@ -222,7 +219,41 @@ void rep_batch_kernel(RepBatchParams p) {
         const unsigned got = atomicAdd(p.head, lane == 0 ? 1u : 0u);
         idx0 = nwaves + __builtin_amdgcn_readlane(got, 0);  // wave-uniform
     }
-    rep_finish_wave(p, lane);
+}
+
+// One block of 256 threads per task (grid-stride over tasks): sum the task's
+// item partials (items of a task are contiguous) in double-double in a fixed
+// order -- thread t takes items t, t+256, ... with several loads in flight, then
+// the wave butterflies and the four waves in order -- scale, write; block 0
+// resets the item counter for the next launch.  (A single 30-photon task has
+// ~2000 items, the double-double n = 30 sweep ~16000: one wave per task took
+// 11.7 us and 80 us of dependent loads for them.)
+__global__ __launch_bounds__(kReduceThreads) void rep_reduce_kernel(RepBatchParams p) {
+    __shared__ ddc red[kReduceThreads / kWave];
+    const unsigned ntasks = to_const(p.counts)[1];
+    for (unsigned task = blockIdx.x; task < ntasks; task += gridDim.x) {
+        const RepTask& T = p.tasks[task];
+        ddc s = {0.0, 0.0, 0.0, 0.0};
+        const uint32_t end = T.item0 + T.nitems;
+#pragma unroll 4
+        for (uint32_t i = T.item0 + threadIdx.x; i < end; i += kReduceThreads) {
+            const ddc q = p.partials[i];
+            ddc_add(s, q);
+        }
+        s = block_reduce(s, red);
+        if (threadIdx.x == 0) {
+            s = ddc_scale(s, T.scale);
+            double* o = p.out + 6 * (size_t)T.out_index;
+            o[0] = s.hr + s.lr;
+            o[1] = s.hi + s.li;
+            o[2] = s.hr;
+            o[3] = s.lr;
+            o[4] = s.hi;
+            o[5] = s.li;
+        }
+        __syncthreads();  // `red` is reused by the next task
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p.head = 0u;
 }
 
 // Slot ranges of the batch kernel instantiations.
@@ -255,6 +286,13 @@ hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks,
         case 4: hipLaunchKernelGGL((rep_batch_kernel<33, 40>), dim3(grid_blocks), dim3(256), 0, st, p); break;
         default: return hipErrorInvalidValue;
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_rep_reduce(p, st);
+}
+
+hipError_t launch_rep_reduce(const RepBatchParams& p, hipStream_t st) {
+    hipLaunchKernelGGL(rep_reduce_kernel, dim3(kReduceBlocks), dim3(kReduceThreads), 0, st, p);
     return hipGetLastError();
 }
 

@@ -61,9 +61,6 @@ struct RepTask {
     uint32_t out_index;    // result slot
     uint32_t nchunks;      // prod_high (m_i + 1)
     double scale;          // 2^-(N-1)
-    double* out;           // the task's 6-double result (device)
-    unsigned* done;        // the task's finished-item count, zero on entry (device); the wave
-                           // finishing the last item sums the task and resets it
     uint32_t hrow[kMaxDigits];  // row (in A) of high digit h
     uint32_t hrad[kMaxDigits];  // radix m_h + 1 of high digit h (digit 0 varies fastest)
     uint32_t lrad[kMaxDigits];  // radix m_d + 1 of low digit d (row 1 + d); read by the
@@ -84,19 +81,21 @@ struct RepBatchParams {
     const double* binom;    // binomial table
     ddc* partials;          // one per item
     double* out;            // 6 doubles per task result
-    unsigned* head;         // batch header: head[0] item counter, zero on entry (the last
-                            // wave to leave resets it); head[kHeadWavesDone] waves that left
-                            // the item loop (zero on entry, reset likewise)
+    unsigned* head;         // item counter, zero on entry; the reduce kernel resets it
     const unsigned* counts; // device: counts[0] = items, counts[1] = tasks (read by the
                             // kernels, so a captured graph serves every batch that fits)
 };
-constexpr int kHeadWavesDone = 32;  // byte offset 128 of the header
+
+constexpr int kReduceBlocks = 512;    // fixed grid of the per-task reduction
+constexpr int kReduceThreads = 256;  // one block of four waves per task
 
 // Kernel instantiations cover slot ranges; rep_range_for() picks the narrowest
 // one for a batch (-1: the batch needs splitting at 32 slots).
 int rep_range_for(int min_ns, int max_ns);
 const void* rep_batch_kernel_fn(int range);
 hipError_t launch_rep_batch(int range, const RepBatchParams& p, int grid_blocks, hipStream_t st);
+// per-task sum of the item partials (shared by the FP64 and double-double sweeps)
+hipError_t launch_rep_reduce(const RepBatchParams& p, hipStream_t st);
 
 // Double-double sweep over the same tasks and items (precision PQB_PREC_INF).
 // Ranges kDDRange0 .. kDDRange0 + 3 cover up to 8, 16, 32 and 40 slots with a
