@@ -167,6 +167,38 @@ def batched_leg() -> dict:
     }
 
 
+def thirty_photon_task():
+    """(compact matrix, row multiplicities, column multiplicities) of the 30-photon
+    transition: inputs in modes 0..29, 18 distinct output modes with multiplicities
+    1 + Multinomial(12, 1/18), seeded 60-mode Haar unitary (seed 1004)."""
+    from piquassoboost_amd.sampling.utilities import haar_unitary
+
+    rng = np.random.default_rng(1004)
+    U = haar_unitary(60, 1004)
+    modes = np.sort(rng.choice(60, size=18, replace=False))
+    rows = (1 + rng.multinomial(12, np.full(18, 1.0 / 18))).astype(np.int64)
+    cols = np.ones(30, dtype=np.int64)
+    return np.ascontiguousarray(U[np.ix_(modes, np.arange(30))]), rows, cols
+
+
+def walk_sweep(walks=(24, 48, 72, 144, 216, 432), reps: int = 30) -> dict:
+    """The 30-photon transition at several walk lengths (Gray steps per chunk): the
+    walk the digit packing gives for each target, the chunk count, and the best of two
+    averages of `reps` replays of the captured launch."""
+    from piquassoboost_amd import _capi as C
+
+    A, rows, cols = thirty_photon_task()
+    ref = C.permanent_repeated(A, rows, cols)
+    out = []
+    for L in walks:
+        p = C.repeated_plan(A, rows, cols, L)
+        v = C.permanent_batch([(A, rows, cols)], walk_length=L)[0]
+        ms = min(C.batch_replay_ms(reps) / reps, C.batch_replay_ms(reps) / reps)
+        out.append({"target": L, "walk": p["walk"], "chunks": p["chunks"], "chunk_digits": p["high"],
+                    "replay_ms": ms, "rel_diff_to_default": abs(v - ref) / abs(ref)})
+    return {"workload": "30 photons in 60 modes, 18 distinct outputs (Haar seed 1004)", "walks": out}
+
+
 def repeated_leg(reps: int = 10) -> dict:
     """One repeated-row permanent, 30 photons in 60 modes -- inputs in modes
     0..29, 18 distinct output modes with multiplicities 1 + Multinomial(12, 1/18), seeded
@@ -174,17 +206,11 @@ def repeated_leg(reps: int = 10) -> dict:
     wall) and per replay of the captured launch; the restated reference sweep (double,
     std::thread) on the host cores beside it."""
     from piquassoboost_amd import _capi as C
-    from piquassoboost_amd.sampling.utilities import haar_unitary
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import oracle
 
-    rng = np.random.default_rng(1004)
-    U = haar_unitary(60, 1004)
-    modes = np.sort(rng.choice(60, size=18, replace=False))
-    rows = (1 + rng.multinomial(12, np.full(18, 1.0 / 18))).astype(np.int64)
-    cols = np.ones(30, dtype=np.int64)
-    A = np.ascontiguousarray(U[np.ix_(modes, np.arange(30))])
+    A, rows, cols = thirty_photon_task()
     plan = C.repeated_plan(A, rows, cols, 1 << 40)
     steps = plan["walk"] * plan["chunks"]
     flops = steps * (2 * 30 + 6 * 29 + 4)
@@ -434,6 +460,9 @@ def main() -> int:
     ap.add_argument("--fma-probe", action="store_true",
                     help="instead of the benchmark, run the FP64 FMA issue-rate probe at 1, 2, 3, 4 and 8 waves per SIMD "
                          "(the FP64 VALU ceiling of the device; profile it for the clock)")
+    ap.add_argument("--walk-sweep", action="store_true",
+                    help="instead of the benchmark, time the 30-photon repeated-row permanent at several walk "
+                         "lengths (replays of the captured launch)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
                          "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
@@ -459,6 +488,11 @@ def main() -> int:
             res[f"waves_per_simd_{w}"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP64_PEAK_TFLOPS}
             print(f"fp64 fma probe, {w} waves/SIMD: {ms:.2f} ms, {tf:.1f} TFLOP/s", file=sys.stderr, flush=True)
         print(json.dumps({"fp64_fma_probe": res, "peak": FP64_PEAK_TFLOPS}), flush=True)
+        return 0
+    if a.walk_sweep:
+        import torch  # noqa: F401  (one HIP runtime: torch's)
+
+        print(json.dumps(walk_sweep()), flush=True)
         return 0
     if a.gpus != 1:
         print("bench.py measures one GPU: the permanent engine runs on a single MI355X in this build "
