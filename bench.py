@@ -36,6 +36,28 @@ TRAFFIC_BYTES_N30 = 773028
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
 
+# Every timed GPU section runs after at least this much of the same work: the clock
+# climbs from ~1.8 to its sustained ~2.3 GHz over the first ~60 ms of FP64 load.
+SUSTAIN_MS = 100.0
+
+
+def sustain(fn, ms: float = SUSTAIN_MS) -> None:
+    """Run fn (GPU work, synchronized after each call) until `ms` of wall time has passed."""
+    import torch
+
+    t0 = time.perf_counter()
+    while True:
+        fn()
+        torch.cuda.synchronize()
+        if (time.perf_counter() - t0) * 1e3 >= ms:
+            return
+
+
+def sustain_replays(C, ms: float = SUSTAIN_MS) -> None:
+    """Replay the last captured batch launch for about `ms` of GPU time."""
+    t = C.batch_replay_ms(3) / 3
+    C.batch_replay_ms(int(min(100000, max(1, np.ceil(ms / max(t, 1e-3))))))
+
 
 def parity_vs_fixture(value: complex, A: np.ndarray, fixture: str, key: str, dd_key: str) -> dict:
     """Relative error of `value` against the stored double-double value of the same
@@ -137,6 +159,7 @@ def batched_leg() -> dict:
         flops += s * (8 * p["slots"] - 2)
     C.permanent_batch(tasks)  # warm: buffers sized for the batch, graph capture
     C.permanent_repeated(*tasks[0])
+    sustain_replays(C)
     t0 = time.perf_counter()
     C.permanent_batch(tasks)
     t_batch = time.perf_counter() - t0
@@ -193,6 +216,7 @@ def walk_sweep(walks=(24, 48, 72, 144, 216, 432), reps: int = 30) -> dict:
     for L in walks:
         p = C.repeated_plan(A, rows, cols, L)
         v = C.permanent_batch([(A, rows, cols)], walk_length=L)[0]
+        sustain_replays(C)
         ms = min(C.batch_replay_ms(reps) / reps, C.batch_replay_ms(reps) / reps)
         out.append({"target": L, "walk": p["walk"], "chunks": p["chunks"], "chunk_digits": p["high"],
                     "replay_ms": ms, "rel_diff_to_default": abs(v - ref) / abs(ref)})
@@ -219,6 +243,7 @@ def repeated_leg(reps: int = 10) -> dict:
     for _ in range(reps):
         C.permanent_repeated(A, rows, cols)
     call_ms = (time.perf_counter() - t0) / reps * 1e3
+    sustain_replays(C)
     replay_ms = C.batch_replay_ms(reps) / reps
     threads = host_cpus()["threads"]
     t0 = time.perf_counter()
@@ -260,8 +285,7 @@ def grad_leg(reps: int = 10) -> dict:
     stream = torch.cuda.current_stream(dev)
 
     def event_ms(fn):
-        fn()
-        torch.cuda.synchronize()
+        sustain(fn)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
@@ -303,6 +327,7 @@ def grad_leg(reps: int = 10) -> dict:
         ones = np.ones(n, np.int64)
         tasks += [(B[b], ones, ones) for b in range(B.shape[0])]
     C.permanent_batch(tasks)
+    sustain_replays(C)
     rep_ms = C.batch_replay_ms(reps) / reps
 
     leaves = [torch.from_numpy(B).to(dev).requires_grad_(True) for n, B in sorted(blocks.items())]
@@ -311,8 +336,7 @@ def grad_leg(reps: int = 10) -> dict:
         loss = sum(torch.log(permanent(A).abs() ** 2).sum() for A in leaves)
         loss.backward()
 
-    step()
-    torch.cuda.synchronize()
+    sustain(step)
     t0 = time.perf_counter()
     for _ in range(reps):
         step()
@@ -428,6 +452,7 @@ def inf_leg(fp64_ms_n30: float, reps: int = 5) -> dict:
     for n, m, seed in ((16, 32, 1001), (20, 40, 1001), (30, M_MODES, SEED)):
         A = np.ascontiguousarray(haar_unitary(m, seed)[:n, :n])
         v = C.permanent(A, C.ALGO_BBFG, C.PREC_INF)
+        sustain_replays(C)
         ms = C.batch_replay_ms(reps) / reps
         out[f"n{n}_ms"] = ms
         out[f"n{n}_perm"] = [v.real, v.imag]
