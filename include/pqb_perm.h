@@ -318,7 +318,8 @@ int pqb_time_perm_device(const double* dA, int64_t rows, int64_t cols, double* d
                          int iters, double* ms);
 
 /* FP64 issue-rate probe (measurement only): `waves_per_simd` waves on every
- * SIMD each run `iters` x 16 independent v_fma_f64 chains; reports the timed
+ * SIMD each run 16 independent v_fma_f64 chains of 8 x `iters` FMAs
+ * (1 <= iters <= 2^30); reports the timed
  * launch's milliseconds (HIP events on the engine stream) and its FP64
  * TFLOP/s (2 flops per FMA).  The ceiling any FP64 VALU kernel can reach on
  * this device at that occupancy.  No reference counterpart. */

@@ -489,7 +489,7 @@ def permanent_minors_batch(A) -> np.ndarray:
     return out
 
 
-def probe_fp64_fma(waves_per_simd: int, iters: int = 1 << 19) -> tuple[float, float]:
+def probe_fp64_fma(waves_per_simd: int, iters: int = 1 << 16) -> tuple[float, float]:
     """(ms, TFLOP/s) of the FP64 FMA issue-rate probe at that occupancy."""
     ms = ctypes.c_double(0.0)
     tf = ctypes.c_double(0.0)

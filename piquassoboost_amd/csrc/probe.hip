@@ -1,7 +1,10 @@
 // FP64 issue-rate probe (measurement only): pqb_probe_fp64_fma.
 //
 // Every lane runs 16 independent chains of v_fma_f64 (no loads in the loop),
-// so the only limits are the FP64 pipe's issue rate and the clock.  Launched
+// eight dependent FMAs per chain per loop iteration (a loop body of 128 FMAs:
+// with one FMA per chain per iteration the loop's branch and counter cost a
+// third of a single wave's issue slots), so the only limits are the FP64
+// pipe's issue rate and the clock.  Launched
 // as exactly `waves_per_simd` waves on every SIMD of the device (blocks of 4
 // waves, `waves_per_simd` blocks per CU), timed with HIP events on the engine
 // stream.  The achieved rate is what any FP64 VALU kernel can reach on this
@@ -20,14 +23,19 @@ namespace pqb {
 
 constexpr int kProbeChains = 16;
 
-__global__ __launch_bounds__(256) void fp64_fma_probe_kernel(double a, double b, int64_t iters, double* out) {
+constexpr int kProbeDepth = 8;  // dependent FMAs per chain per loop iteration
+
+__global__ __launch_bounds__(256) void fp64_fma_probe_kernel(double a, double b, int iters, double* out) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double x[kProbeChains];
 #pragma unroll
     for (int c = 0; c < kProbeChains; ++c) x[c] = 1.0 + 1e-3 * (double)(threadIdx.x & 7) + 1e-4 * c;
-    for (int64_t i = 0; i < iters; ++i) {
+    for (int i = 0; i < iters; ++i) {
 #pragma unroll
-        for (int c = 0; c < kProbeChains; ++c) x[c] = fma(x[c], a, b);
+        for (int u = 0; u < kProbeDepth; ++u) {
+#pragma unroll
+            for (int c = 0; c < kProbeChains; ++c) x[c] = fma(x[c], a, b);
+        }
     }
     double s = 0.0;
 #pragma unroll
@@ -56,7 +64,7 @@ static int probe_entry(int waves_per_simd, int64_t iters, double* ms, double* tf
         if (rep == 1) e = hipEventRecord(e0, E.stream);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(fp64_fma_probe_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, E.stream,
-                           1.0 - 1e-9, 1e-9, iters, dout);
+                           1.0 - 1e-9, 1e-9, (int)iters, dout);
         e = hipGetLastError();
         if (e == hipSuccess && rep == 1) e = hipEventRecord(e1, E.stream);
     }
@@ -67,7 +75,7 @@ static int probe_entry(int waves_per_simd, int64_t iters, double* ms, double* tf
     if (e1) (void)hipEventDestroy(e1);
     if (e != hipSuccess) return hip_fail(e, "fp64 probe");
     *ms = f;
-    const double flops = (double)blocks * kBlock * (double)iters * kProbeChains * 2.0;
+    const double flops = (double)blocks * kBlock * (double)iters * kProbeDepth * kProbeChains * 2.0;
     *tflops = flops / ((double)f * 1e-3) / 1e12;
     return PQB_OK;
 }
