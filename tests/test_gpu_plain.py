@@ -225,6 +225,29 @@ def test_largest_size_n40_block_diagonal():
     assert err < 1e-8
 
 
+@pytest.mark.parametrize("n1,n2", [(16, 15), (16, 16), (17, 16), (18, 18)])
+def test_block_diagonal_past_30_columns(n1, n2):
+    """The sweep instantiations past n = 30 (31 and 32 columns at three waves per SIMD,
+    32 with the single-pair step loop; 33 and 36 at one wave per SIMD): a block-diagonal
+    of two Haar blocks with rows and columns permuted has perm = perm(B1) perm(B2), the
+    factors from the double-double oracle."""
+    from piquassoboost_amd.sampling.utilities import haar_unitary
+
+    n = n1 + n2
+    B1 = haar_unitary(2 * n1, 2100 + n)[:n1, :n1]
+    B2 = haar_unitary(2 * n2, 2200 + n)[:n2, :n2]
+    A = np.zeros((n, n), dtype=np.complex128)
+    A[:n1, :n1] = B1
+    A[n1:, n1:] = B2
+    rng = np.random.default_rng(2300 + n)
+    A = A[rng.permutation(n)][:, rng.permutation(n)]
+    ref = oracle.bbfg(B1, "dd") * oracle.bbfg(B2, "dd")
+    got = C.permanent(np.ascontiguousarray(A), C.ALGO_BBFG, C.PREC_FP64)
+    err = rel_err(got, ref)
+    print(f"n={n}: rel_err={err:.3e}", flush=True)
+    assert err < 1e-10
+
+
 def test_exact_small_integer_matrices():
     rng = np.random.default_rng(21)
     for n in range(1, 15):
