@@ -88,6 +88,7 @@ SIGNATURES = {
 
 _lib = None
 _lock = threading.Lock()
+_C128 = np.dtype(np.complex128)
 
 
 class EngineError(Exception):
@@ -265,31 +266,47 @@ def _task_array(tasks):
     arrays it points into (kept alive by the caller).  Matrices whose rows are
     contiguous complex128 (see as_rows) and C-contiguous int64 states are used
     in place."""
-    mats, lds, rows, cols = [], [], [], []
-    for i, (A, r, c) in enumerate(tasks):
-        M, ld = as_rows(A)
-        rv = as_state(r)
-        cv = as_state(c)
-        if rv.shape[0] != M.shape[0] or cv.shape[0] != M.shape[1]:
-            raise Exception(
-                f"task {i}: state lengths must match the matrix: len(output_state) == rows, len(input_state) == cols"
-            )
+    n = len(tasks)
+    mats, lds, shapes, rows, cols = [], [], [], [], []
+    for A, r, c in tasks:
+        # C-contiguous complex128 matrices (the common case) skip as_rows
+        if type(A) is np.ndarray and A.dtype == _C128 and A.ndim == 2 and A.flags.c_contiguous and A.size:
+            M, ld = A, A.shape[1]
+        else:
+            M, ld = as_rows(A)
         mats.append(M)
         lds.append(ld)
-        rows.append(rv)
-        cols.append(cv)
-    n = len(mats)
-    nr = np.fromiter((v.shape[0] for v in rows), dtype=np.int64, count=n)
-    nc = np.fromiter((v.shape[0] for v in cols), dtype=np.int64, count=n)
-    # the occupation numbers of all tasks packed into two arrays: a pointer per
-    # task is then base + offset (an address lookup per array costs ~1 us)
-    rall = np.concatenate(rows) if n else np.zeros(0, np.int64)
-    call = np.concatenate(cols) if n else np.zeros(0, np.int64)
+        shapes.append(M.shape)
+        rows.append(r)
+        cols.append(c)
+    shapes = np.array(shapes, dtype=np.int64).reshape(n, 2)
+    # the occupation numbers of all tasks packed into two int64 arrays (one
+    # conversion pass each): a pointer per task is then base + offset
+    try:
+        rall = np.concatenate(rows, dtype=np.int64, casting="same_kind") if n else np.zeros(0, np.int64)
+        call = np.concatenate(cols, dtype=np.int64, casting="same_kind") if n else np.zeros(0, np.int64)
+        nr = np.fromiter(map(len, rows), dtype=np.int64, count=n)
+        nc = np.fromiter(map(len, cols), dtype=np.int64, count=n)
+        if rall.ndim != 1 or call.ndim != 1 or int(nr.sum()) != rall.shape[0] or int(nc.sum()) != call.shape[0]:
+            raise ValueError
+    except (ValueError, TypeError):
+        rows = [as_state(r) for r in rows]  # raises the reference's message for a bad state
+        cols = [as_state(c) for c in cols]
+        rall = np.concatenate(rows) if n else np.zeros(0, np.int64)
+        call = np.concatenate(cols) if n else np.zeros(0, np.int64)
+        nr = np.fromiter((v.shape[0] for v in rows), dtype=np.int64, count=n)
+        nc = np.fromiter((v.shape[0] for v in cols), dtype=np.int64, count=n)
+    bad = np.flatnonzero((nr != shapes[:, 0]) | (nc != shapes[:, 1]))
+    if bad.size:
+        raise Exception(
+            f"task {int(bad[0])}: state lengths must match the matrix: len(output_state) == rows, "
+            "len(input_state) == cols"
+        )
     arr = np.empty((n, 6), dtype=np.int64)
     arr[:, 0] = np.fromiter((_address(M) for M in mats), dtype=np.int64, count=n)
     arr[:, 1] = nr
     arr[:, 2] = nc
-    arr[:, 3] = np.fromiter(lds, dtype=np.int64, count=n)
+    arr[:, 3] = lds
     arr[:, 4] = rall.__array_interface__["data"][0] + 8 * (np.cumsum(nr) - nr)
     arr[:, 5] = call.__array_interface__["data"][0] + 8 * (np.cumsum(nc) - nc)
     return arr, (mats, rall, call)

@@ -442,3 +442,45 @@ def test_row_stride_binding_host_only():
     assert (ld, stride) == (16, 192) and B.ctypes.data == S.ctypes.data
     B, ld, stride = C._stack_strides(S[:, :, ::2])
     assert (ld, stride) == (8, 96) and B.flags.c_contiguous
+
+
+def _read_i64(addr, n):
+    import ctypes
+
+    return np.ctypeslib.as_array(ctypes.cast(int(addr), ctypes.POINTER(ctypes.c_int64)), shape=(n,)).copy()
+
+
+def test_task_array_marshalling():
+    """The pqb_task array built for pqb_perm_batch: every field of every task, for
+    matrices used in place (C-contiguous complex128, a row slice with ld > cols) or
+    converted (float64, a column slice), and states given as int32 arrays or lists."""
+    rng = np.random.default_rng(7)
+    big = rng.standard_normal((6, 9)) + 1j * rng.standard_normal((6, 9))
+    tasks = [
+        (np.ascontiguousarray(big[:3, :4]), np.array([1, 2, 1]), np.array([1, 1, 1, 1])),
+        (big[1:4, 2:6], np.array([2, 1, 1], dtype=np.int32), [1, 1, 1, 1]),  # rows of `big`: ld = 9
+        (rng.standard_normal((2, 2)), [1, 1], np.array([1, 1], dtype=np.int64)),  # float64: copied
+        (big[:, ::2][:2, :3], np.array([2, 1]), np.array([1, 1, 1])),  # column stride 2: copied
+    ]
+    arr, keep = C._task_array(tasks)
+    assert arr.shape == (4, 6)
+    want_ld = [4, 9, 2, 3]
+    for k, (A, r, c) in enumerate(tasks):
+        rows, cols = np.shape(A)
+        assert arr[k, 1] == rows and arr[k, 2] == cols and arr[k, 3] == want_ld[k]
+        assert (_read_i64(arr[k, 4], rows) == np.asarray(r)).all()
+        assert (_read_i64(arr[k, 5], cols) == np.asarray(c)).all()
+    # task 1 is passed in place: its address is that of big[1, 2]
+    assert arr[1, 0] == big.__array_interface__["data"][0] + 16 * (1 * 9 + 2)
+    del keep
+
+
+def test_task_array_errors():
+    ok = (np.eye(2, dtype=np.complex128), np.array([1, 1]), np.array([1, 1]))
+    with pytest.raises(Exception, match="task 1: state lengths must match"):
+        C._task_array([ok, (np.eye(3, dtype=np.complex128), np.array([1, 1]), np.array([1, 1, 1]))])
+    with pytest.raises(Exception, match="numpy2PicState_int64: Wrong matrix dimension was given"):
+        C._task_array([ok, (np.eye(2, dtype=np.complex128), np.ones((2, 1), dtype=np.int64), np.array([1, 1]))])
+    col = np.ones((2, 1), dtype=np.int64)  # every state 2-D: still refused
+    with pytest.raises(Exception, match="numpy2PicState_int64: Wrong matrix dimension was given"):
+        C._task_array([(np.eye(2, dtype=np.complex128), col, col)] * 2)
