@@ -31,7 +31,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 # HBM bytes per launch of the n=30 sweep: FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a
 # 256 MiB copy kernel), measured with rocprofv3 --pmc on this workload
 # (profiles/plain_sweep_n30_counters.md).
-TRAFFIC_BYTES_N30 = 773028
+TRAFFIC_BYTES_N30 = 863296
 
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
