@@ -23,6 +23,28 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
 
 
+# Worst relative error of each parity check of a run, printed after the tests so that
+# the per-result errors are in the run's summary and not only in captured output.
+PARITY = {}
+
+
+def record_parity(name, err, bar):
+    """Keep the worst `err` seen for check `name` (with its bar) for the run summary."""
+    err = float(err)
+    prev = PARITY.get(name)
+    if prev is None or err > prev[0]:
+        PARITY[name] = (err, float(bar))
+
+
+def pytest_terminal_summary(terminalreporter, exitstatus, config):
+    if not PARITY:
+        return
+    terminalreporter.write_sep("-", "parity: worst relative error of each check (bar)")
+    for name in sorted(PARITY):
+        err, bar = PARITY[name]
+        terminalreporter.write_line(f"{name}: {err:.2e} (bar {bar:.0e})")
+
+
 @pytest.fixture(scope="session")
 def kat_cases():
     cases = json.loads((GOLDEN / "kat.json").read_text())

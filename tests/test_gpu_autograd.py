@@ -13,6 +13,8 @@
 import numpy as np
 import pytest
 
+from conftest import record_parity
+
 torch = pytest.importorskip("torch")
 
 from piquassoboost_amd.autograd import permanent  # noqa: E402
@@ -139,3 +141,4 @@ def test_clements_chain_finite_differences():
                 worst = max(worst, err)
                 assert err < 1e-5, (m, which, int(k), fd, g)
     print(f"autograd chain finite differences: worst {worst:.3e}", flush=True)
+    record_parity("config 5: Clements phases through autograd vs central differences", worst, 1e-5)

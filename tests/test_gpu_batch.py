@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import GOLDEN, rel_err
+from conftest import GOLDEN, record_parity, rel_err
 from piquassoboost_amd import _capi as C
 from piquassoboost_amd.sampling.utilities import adapt_interferometer, uneven_batch
 
@@ -80,6 +80,7 @@ def test_uneven_batch_every_result_vs_fixture(uneven_fixture):
         photons = int(ins[k].sum())
         print(f"{name}: 4096 results, worst rel_err={err[k]:.3e} (task {k}, {photons} photons), "
               f"median {np.median(err):.2e}", flush=True)
+        record_parity(f"config 4: 4096 tasks, {name}, vs dd fixture", err[k], 1e-10)
         assert err[k] < 1e-10, (name, k)
 
 
@@ -90,6 +91,7 @@ def test_uneven_batch_every_result_per_call_vs_fixture(uneven_fixture):
     got = [C.permanent_repeated(adapt_interferometer(U, i, o), o[o > 0], i[i > 0]) for i, o in zip(ins, outs)]
     err, k = _worst(got, ref)
     print(f"pqb_perm_repeated per call: worst rel_err={err[k]:.3e} (task {k})", flush=True)
+    record_parity("config 4: 4096 tasks, per call, vs dd fixture", err[k], 1e-10)
     assert err[k] < 1e-10
 
 

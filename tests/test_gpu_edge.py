@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import random_complex, rel_err
+from conftest import random_complex, record_parity, rel_err
 from piquassoboost_amd import _capi as C
 
 pytestmark = pytest.mark.gpu
@@ -56,4 +56,5 @@ def test_many_tiny_tasks():
         errs.append(rel_err(got[k], oracle.bbfg(A, "dd")))
     worst = max(errs)
     print(f"20000 tiny tasks, {len(errs)} checked: worst rel_err={worst:.3e}", flush=True)
+    record_parity("batch: 20000 tasks of 1..6 photons vs oracle dd", worst, 1e-12)
     assert worst < 1e-12

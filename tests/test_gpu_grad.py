@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import GOLDEN, random_complex, rel_err
+from conftest import GOLDEN, random_complex, record_parity, rel_err
 from piquassoboost_amd import _capi as C
 from piquassoboost_amd.sampling.utilities import gradient_blocks
 
@@ -119,6 +119,7 @@ def test_grad_workload_n26_row_col_minors_vs_fixture(grad_blocks, minors_fixture
     worst = max(err_r.max(), err_c.max())
     print(f"grad n=26, 8 blocks x 51 minors: worst rel_err={worst:.3e} (median {np.median(err_r):.2e})",
           flush=True)
+    record_parity("config 5: n=26 gradients vs oracle minors", worst, 1e-9)
     assert worst < 1e-9
     # host batch entry on the same blocks: same values to the bit (same plan)
     _, Gh = C.permanent_grad_batch(B)
@@ -133,6 +134,7 @@ def test_grad_workload_n20_all_minors_vs_fixture(minors_fixture):
     err = np.abs(G - f["m20"]) / np.abs(f["m20"])
     print(f"grad n=20, 16 blocks x 400 minors: worst rel_err={err.max():.3e} (median {np.median(err):.2e})",
           flush=True)
+    record_parity("config 5: n=20 gradients vs oracle minors", err.max(), 1e-9)
     assert err.max() < 1e-9
 
 
@@ -178,6 +180,8 @@ def test_grad_finite_differences(grad_blocks):
         anl = (-2 * (np.conj(p) * G[i, j]).imag if imag else 2 * (np.conj(p) * G[i, j]).real) / abs(p) ** 2
         worst_l = max(worst_l, abs(fdl - anl) / max(abs(anl), 1e-3))
     print(f"grad finite differences: perm worst rel {worst_p:.3e}, loss worst {worst_l:.3e}", flush=True)
+    record_parity("config 5: central differences of perm", worst_p, 1e-8)
+    record_parity("config 5: central differences of the loss", worst_l, 1e-6)
     assert worst_p < 1e-8
     assert worst_l < 1e-6
 

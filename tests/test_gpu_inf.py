@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import random_complex, rel_err
+from conftest import random_complex, record_parity, rel_err
 from exact_sum import exact_bbfg, exact_repeated, expand
 from piquassoboost_amd import _capi as C
 from piquassoboost_amd.sampling import Boson_Sampling_Utilities as BSU
@@ -162,6 +162,7 @@ def test_haar_n28_n30_match_the_double_double_fixture(haar_blocks, name):
     ref = complex(h[0] + h[1], h[2] + h[3])
     got = inf(A)
     print(f"{name}: got={got!r} dd fixture={ref!r}", flush=True)
+    record_parity(f"double-double sweep: {name[4:]} vs dd fixture (bit-exact)", rel_err(got, ref), 0)
     assert got == ref
 
 

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import GOLDEN, random_complex, rel_err
+from conftest import GOLDEN, random_complex, record_parity, rel_err
 from exact_sum import exact_bbfg
 from piquassoboost_amd import _capi as C
 from piquassoboost_amd.sampling import Boson_Sampling_Utilities as BSU
@@ -30,6 +30,7 @@ def test_square_random_vs_oracle(n):
     ref = oracle.bbfg(A, "dd")
     got = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
     print(f"n={n} rel_err={rel_err(got, ref):.3e}", flush=True)
+    record_parity("plain FP64, random n=1..24 vs oracle dd", rel_err(got, ref), TOL_SMALL)
     assert rel_err(got, ref) < TOL_SMALL
 
 
@@ -164,6 +165,7 @@ def test_haar_n20(haar_blocks):
         got = lib_cls(A).calculate()
         err = rel_err(got, ref)
         print(f"n=20 Haar {lib_cls.__name__}: rel_err={err:.3e}", flush=True)
+        record_parity("config 1: n=20 Haar vs Glynn long double", err, 1e-12)
         assert err < 1e-12
 
 
@@ -177,6 +179,7 @@ def test_haar_n28_n30(haar_blocks, name):
     got = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
     err = rel_err(got, ref)
     print(f"{name}: got={got!r} ref={ref!r} rel_err={err:.3e}", flush=True)
+    record_parity(f"config 2a: {name[4:]} Haar vs dd fixture", err, 1e-10)
     assert err < 1e-10
 
 
@@ -222,6 +225,7 @@ def test_largest_size_n40_block_diagonal():
     got = C.permanent(A, C.ALGO_BBFG, C.PREC_FP64)
     err = rel_err(got, ref)
     print(f"n=40: got={got!r} ref={ref!r} rel_err={err:.3e}", flush=True)
+    record_parity("plain FP64, n=40 block diagonal", err, 1e-8)
     assert err < 1e-8
 
 
@@ -245,6 +249,7 @@ def test_block_diagonal_past_30_columns(n1, n2):
     got = C.permanent(np.ascontiguousarray(A), C.ALGO_BBFG, C.PREC_FP64)
     err = rel_err(got, ref)
     print(f"n={n}: rel_err={err:.3e}", flush=True)
+    record_parity("plain FP64, n=31..36 block diagonal", err, 1e-10)
     assert err < 1e-10
 
 
@@ -283,6 +288,7 @@ def test_bernoulli26_exact():
         assert v == exact
         assert dv == float(exact)
         got = BSU.BBFGPermanentDouble(A).calculate()
+        record_parity("config 2b: n=26 Bernoulli, integer result vs oracle (bit-exact)", abs(v - exact) / exact, 0)
         assert got.real == float(exact)
 
 

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import GOLDEN, random_complex, rel_err
+from conftest import GOLDEN, random_complex, record_parity, rel_err
 from piquassoboost_amd import _capi as C
 from piquassoboost_amd.sampling import Boson_Sampling_Utilities as BSU
 from piquassoboost_amd.sampling.permanent_calculators import (
@@ -28,6 +28,7 @@ def test_30_photons_18_outputs_golden():
     got = C.permanent_repeated(d["A"], d["row_mult"], d["col_mult"])
     err = rel_err(got, ref)
     print(f"30 photons / 18 outputs, compact: rel_err={err:.3e}", flush=True)
+    record_parity("config 3: 30 photons, compact, vs dd fixture", err, 1e-10)
     assert err < 1e-10
     # the same transition on the full 60-mode unitary with zero multiplicities
     full = BSU.BBFGRepeatedPermanentCalculatorDouble(d["U"], d["input_state"], d["output_state"]).calculate()
@@ -39,6 +40,7 @@ def test_30_photons_18_outputs_golden():
     plain = C.permanent(E, C.ALGO_BBFG, C.PREC_FP64)  # = the repeated value (no multiplicity factor)
     err = rel_err(plain, ref)
     print(f"30 photons / 18 outputs, expanded 30 x 30 through the plain sweep: rel_err={err:.3e}", flush=True)
+    record_parity("config 3: 30 photons, expanded 30x30, vs dd fixture", err, 1e-10)
     assert err < 1e-10
 
 
