@@ -325,6 +325,13 @@ int pqb_time_perm_device(const double* dA, int64_t rows, int64_t cols, double* d
  * this device at that occupancy.  No reference counterpart. */
 int pqb_probe_fp64_fma(int waves_per_simd, int64_t iters, double* ms, double* tflops);
 
+/* Clock stamp (measurement only): a one-wave launch on `stream` writes the
+ * shader-clock counter (s_memtime) and the 100 MHz real-time counter
+ * (s_memrealtime) to d_dst[0], d_dst[1] (device memory, 16 bytes).  Stamps taken
+ * before and after some work on the same stream give its average clock,
+ * (c1 - c0) / (r1 - r0) x 100 MHz.  No reference counterpart. */
+int pqb_clock_stamp(unsigned long long* d_dst, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,7 @@ SIGNATURES = {
     "pqb_perm_batch_device_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                        ctypes.POINTER(ctypes.c_size_t)]),
     "pqb_probe_fp64_fma": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _dp, _dp]),
+    "pqb_clock_stamp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "pqb_perm_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "pqb_perm_device_noalloc_scratch_bytes": (ctypes.c_size_t, []),
@@ -504,6 +505,11 @@ def permanent_minors_batch(A) -> np.ndarray:
     _check(lib().pqb_perm_minors_batch(M.ctypes.data_as(_dp), count, rows, cols, ld, stride,
                                        out.ctypes.data_as(_dp)))
     return out
+
+
+def clock_stamp(d_dst_ptr: int, stream: int = 0) -> None:
+    """Asynchronously write the (shader clock, 100 MHz real-time) counters to d_dst (2 x u64)."""
+    _check(lib().pqb_clock_stamp(d_dst_ptr, stream or None))
 
 
 def probe_fp64_fma(waves_per_simd: int, iters: int = 1 << 16) -> tuple[float, float]:

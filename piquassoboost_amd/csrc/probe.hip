@@ -80,7 +80,27 @@ static int probe_entry(int waves_per_simd, int64_t iters, double* ms, double* tf
     return PQB_OK;
 }
 
+// One lane writes the shader-clock counter (s_memtime) and the 100 MHz
+// real-time counter (s_memrealtime).  A stamp before and one after some work on
+// the same stream give the average shader clock over that work.  Block 0 of a
+// launch goes to the same XCD every time, so both stamps read one counter.
+__global__ void clock_stamp_kernel(unsigned long long* dst) {
+    if (threadIdx.x == 0) {
+        const unsigned long long c = __builtin_amdgcn_s_memtime();
+        const unsigned long long r = __builtin_amdgcn_s_memrealtime();
+        dst[0] = c;
+        dst[1] = r;
+    }
+}
+
 }  // namespace pqb
+
+extern "C" int pqb_clock_stamp(unsigned long long* d_dst, void* stream) {
+    if (!d_dst) return pqb::fail(PQB_ERR_ARG, "NULL device pointer");
+    hipLaunchKernelGGL(pqb::clock_stamp_kernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), d_dst);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PQB_OK : pqb::hip_fail(e, "clock stamp launch");
+}
 
 extern "C" int pqb_probe_fp64_fma(int waves_per_simd, int64_t iters, double* ms, double* tflops) {
     try {
