@@ -522,7 +522,7 @@ def main() -> int:
         print(json.dumps(walk_sweep()), flush=True)
         return 0
     if a.gpus != 1:
-        print("bench.py measures one GPU: the permanent engine runs on a single MI355X in this build "
+        print("bench.py measures one GPU: the permanent engine runs on a single MI355X "
               "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)
         return 2
 

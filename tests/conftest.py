@@ -1,6 +1,6 @@
 """Shared test configuration.
 
-Markers: ``gpu`` -- needs a visible MI355X (run with ``-m gpu`` on the GPU box).
+Markers: ``gpu`` -- needs a visible MI355X (run with ``-m gpu`` on a machine that has one).
 Everything unmarked runs on a CPU-only machine: the oracle against the golden
 fixtures and closed forms, host-side logic, and that the C-ABI library loads
 and exports every symbol of include/pqb_perm.h.
