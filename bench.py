@@ -470,6 +470,71 @@ def inf_leg(fp64_ms_n30: float, reps: int = 5) -> dict:
     return out
 
 
+def _smi_sample():
+    """(socket power in W, graphics clock in MHz) of GPU 0 from amd-smi; (None, None)
+    where the tool or a field is missing."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["amd-smi", "metric", "--power", "--clock", "--json"], capture_output=True,
+                             text=True, timeout=10).stdout
+        d = json.loads(out)
+        g = (d["gpu_data"] if isinstance(d, dict) else d)[0]
+        return float(g["power"]["socket_power"]["value"]), float(g["clock"]["gfx_0"]["clk"]["value"])
+    except Exception:
+        return None, None
+
+
+def power_leg(secs: float) -> dict:
+    """Socket power and graphics clock while the n=30 sweep runs back to back for `secs`
+    seconds, then the FP64 FMA probe at the sweep's three waves per SIMD: samples every
+    ~0.5 s after the first 2 s (the clock and power settle within ~0.1 s)."""
+    import threading
+
+    import torch
+
+    from piquassoboost_amd import _capi as C
+    from piquassoboost_amd.sampling.utilities import haar_unitary
+
+    dev = torch.device("cuda:0")
+    A = np.ascontiguousarray(haar_unitary(M_MODES, SEED)[:N, :N])
+    dA = torch.from_numpy(A.view(np.float64).copy()).to(dev)
+    d_out = torch.zeros(6, dtype=torch.float64, device=dev)
+
+    def run(work):
+        samples, stop = [], threading.Event()
+
+        def sampler():
+            if not stop.wait(2.0):
+                while not stop.is_set():
+                    samples.append(_smi_sample())
+                    stop.wait(0.5)
+
+        th = threading.Thread(target=sampler)
+        th.start()
+        t0 = time.perf_counter()
+        rates = []
+        while time.perf_counter() - t0 < secs:
+            rates.append(work())  # ctypes releases the GIL inside the engine call
+        stop.set()
+        th.join()
+        pw = [p for p, _ in samples if p is not None]
+        mhz = [c for _, c in samples if c is not None]
+        return {"socket_power_w": pw, "gfx_clock_mhz": mhz, "rate": rates[-1] if rates else None}
+
+    def sweep():
+        ms = C.time_permanent_device(dA.data_ptr(), N, N, d_out.data_ptr(), 100) / 100
+        return {"ms_per_perm": ms, "frac_of_peak": flops_plain(N) / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+
+    def probe():
+        ms, tf = C.probe_fp64_fma(3, 1 << 17)
+        return {"tflops": tf, "frac_of_peak": tf / FP64_PEAK_TFLOPS}
+
+    res = {"seconds_each": secs, "n30_sweep": run(sweep), "fp64_fma_probe_3_waves": run(probe)}
+    res["tool"] = "amd-smi metric --power --clock"
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -490,6 +555,9 @@ def main() -> int:
     ap.add_argument("--walk-sweep", action="store_true",
                     help="instead of the benchmark, time the 30-photon repeated-row permanent at several walk "
                          "lengths (replays of the captured launch)")
+    ap.add_argument("--power", type=float, default=0.0,
+                    help="instead of the benchmark, run the n=30 sweep and then the FP64 FMA probe for this many "
+                         "seconds each and report socket power and graphics clock (amd-smi) during each")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
                          "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
@@ -515,6 +583,10 @@ def main() -> int:
             res[f"waves_per_simd_{w}"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP64_PEAK_TFLOPS}
             print(f"fp64 fma probe, {w} waves/SIMD: {ms:.2f} ms, {tf:.1f} TFLOP/s", file=sys.stderr, flush=True)
         print(json.dumps({"fp64_fma_probe": res, "peak": FP64_PEAK_TFLOPS}), flush=True)
+        return 0
+    if a.power:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        print(json.dumps(power_leg(a.power)), flush=True)
         return 0
     if a.walk_sweep:
         import torch  # noqa: F401  (one HIP runtime: torch's)
