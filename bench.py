@@ -266,35 +266,18 @@ def repeated_leg(reps: int = 10) -> dict:
     }
 
 
-def grad_leg(reps: int = 10) -> dict:
-    """Gradient workload: 256 n=20 and 8 n=26 complex128 blocks, block b the
-    top-left corner of the 50-mode Haar unitary of seed 1006 + b, inputs
-    resident in HBM.  Forward = the plain sweep of
-    the 264 permanents (pqb_perm_batch_device, one launch per size).
-    Forward+backward = the adjoint sweep, which yields the permanents and every
-    d perm / d a_ij (one launch per size plus its reduction).  Both timed with
-    HIP events on the launch stream; the same step through torch.autograd
-    (loss sum log|perm|^2) by wall clock."""
+def grad_workload():
+    """The gradient workload resident on cuda:0: (blocks, stream, forward, sweep), where
+    forward() launches the plain sweep of the 264 permanents (pqb_perm_batch_device, one
+    launch per size) and sweep() the adjoint sweep (permanents and every d perm / d a_ij,
+    one launch per size plus its reduction), both on torch's current stream."""
     import torch
     from piquassoboost_amd import _capi as C
-    from piquassoboost_amd.autograd import permanent
     from piquassoboost_amd.sampling.utilities import gradient_blocks
 
     blocks, _ = gradient_blocks()
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
-
-    def event_ms(fn):
-        sustain(fn)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps
-
     fbufs, gbufs = [], []
     for n, B in sorted(blocks.items()):
         count = B.shape[0]
@@ -318,6 +301,36 @@ def grad_leg(reps: int = 10) -> dict:
         for n, count, dA, dp, dG, work, wb in gbufs:
             C.permanent_grad_device(dA.data_ptr(), count, n, dp.data_ptr(), dG.data_ptr(), work.data_ptr(), wb,
                                     stream.cuda_stream)
+
+    return blocks, stream, forward, sweep
+
+
+def grad_leg(reps: int = 10) -> dict:
+    """Gradient workload: 256 n=20 and 8 n=26 complex128 blocks, block b the
+    top-left corner of the 50-mode Haar unitary of seed 1006 + b, inputs
+    resident in HBM.  Forward = the plain sweep of
+    the 264 permanents (pqb_perm_batch_device, one launch per size).
+    Forward+backward = the adjoint sweep, which yields the permanents and every
+    d perm / d a_ij (one launch per size plus its reduction).  Both timed with
+    HIP events on the launch stream; the same step through torch.autograd
+    (loss sum log|perm|^2) by wall clock."""
+    import torch
+    from piquassoboost_amd import _capi as C
+    from piquassoboost_amd.autograd import permanent
+
+    blocks, stream, forward, sweep = grad_workload()
+    dev = torch.device("cuda:0")
+
+    def event_ms(fn):
+        sustain(fn)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
 
     fwd_ms = event_ms(forward)
     grad_ms = event_ms(sweep)
@@ -486,9 +499,10 @@ def _smi_sample():
 
 
 def power_leg(secs: float) -> dict:
-    """Socket power and graphics clock while the n=30 sweep runs back to back for `secs`
-    seconds, then the FP64 FMA probe at the sweep's three waves per SIMD: samples every
-    ~0.5 s after the first 2 s (the clock and power settle within ~0.1 s)."""
+    """Socket power and graphics clock while each workload runs back to back for `secs`
+    seconds: the n=30 sweep, the FP64 FMA probe at the sweep's three waves per SIMD, and the
+    gradient workload's plain forward and adjoint sweep.  Samples every ~0.5 s after the
+    first 2 s (the clock and power settle within ~0.1 s)."""
     import threading
 
     import torch
@@ -531,6 +545,24 @@ def power_leg(secs: float) -> dict:
         return {"tflops": tf, "frac_of_peak": tf / FP64_PEAK_TFLOPS}
 
     res = {"seconds_each": secs, "n30_sweep": run(sweep), "fp64_fma_probe_3_waves": run(probe)}
+
+    # the gradient workload: its plain forward, then its adjoint sweep
+    _, stream, forward, adjoint = grad_workload()
+
+    def timed(fn, reps=10):
+        def work():
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            return {"ms": e0.elapsed_time(e1) / reps}
+        return work
+
+    res["grad_forward_264"] = run(timed(forward))
+    res["grad_adjoint_264"] = run(timed(adjoint))
     res["tool"] = "amd-smi metric --power --clock"
     return res
 
@@ -556,8 +588,9 @@ def main() -> int:
                     help="instead of the benchmark, time the 30-photon repeated-row permanent at several walk "
                          "lengths (replays of the captured launch)")
     ap.add_argument("--power", type=float, default=0.0,
-                    help="instead of the benchmark, run the n=30 sweep and then the FP64 FMA probe for this many "
-                         "seconds each and report socket power and graphics clock (amd-smi) during each")
+                    help="instead of the benchmark, run the n=30 sweep, the FP64 FMA probe and the gradient "
+                         "workload's forward and adjoint for this many seconds each and report socket power and "
+                         "graphics clock (amd-smi) during each")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="instead of the benchmark, run 10 device-to-device copies of this many MiB "
                          "(reference kernel of known traffic for calibrating FETCH_SIZE / WRITE_SIZE)")
