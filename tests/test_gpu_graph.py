@@ -1,0 +1,118 @@
+"""The device entry points under HIP graph capture (torch.cuda.CUDAGraph): the plain
+permanent (pqb_perm_device), its no-allocation form (pqb_perm_device_noalloc), the batch
+of plain permanents (pqb_perm_batch_device) and the adjoint sweep (pqb_perm_grad_device),
+captured once after an eager warm-up and replayed several times, with eager work and
+synchronisations between the replays.  Every replay must give the eager result bit for
+bit: the launches record no memset (each grid counter is restored to zero by the kernel
+that uses it, the no-allocation entry clears its counter with a kernel), so a replay
+starts from the state the previous one left."""
+
+import numpy as np
+import pytest
+
+from conftest import random_complex
+from piquassoboost_amd import _capi as C
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+REPLAYS = 4
+
+
+def _stream():
+    return torch.cuda.current_stream(DEV).cuda_stream
+
+
+def _capture_and_replay(launch, outputs, scratch):
+    """Run `launch(stream)` eagerly, capture it in a graph, replay it REPLAYS times; the
+    outputs after every replay must equal the eager ones bit for bit.  Between replays
+    the outputs are overwritten by eager kernels and `scratch` is refilled, so a replay
+    that skipped work or read stale memory would show."""
+    side = torch.cuda.Stream(DEV)
+    side.wait_stream(torch.cuda.current_stream(DEV))
+    with torch.cuda.stream(side):
+        for _ in range(2):  # warm-up: one-time device queries happen outside the capture
+            launch(side.cuda_stream)
+    torch.cuda.current_stream(DEV).wait_stream(side)
+    torch.cuda.synchronize()
+    eager = [o.clone() for o in outputs]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        launch(_stream())
+    torch.cuda.synchronize()
+    for rep in range(REPLAYS):
+        for o in outputs:
+            o.fill_(float("nan") if o.is_floating_point() or o.is_complex() else 7)
+        scratch.mul_(1.0)  # unrelated eager work on the default stream
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for o, e in zip(outputs, eager):
+            assert torch.equal(o.view(torch.uint8), e.view(torch.uint8)), f"replay {rep} differs"
+    del g
+    return eager
+
+
+def test_plain_device_in_a_graph():
+    rng = np.random.default_rng(4101)
+    A = random_complex(rng, 20, 20)
+    dA = torch.from_numpy(A).to(DEV)
+    out = torch.zeros(6, dtype=torch.float64, device=DEV)
+    wb = C.device_workspace_bytes(20, 20)
+    work = torch.zeros(wb, dtype=torch.uint8, device=DEV)
+    scratch = torch.ones(1 << 16, dtype=torch.float64, device=DEV)
+    print("graph: pqb_perm_device n=20", flush=True)
+    eager = _capture_and_replay(
+        lambda st: C.permanent_device(dA.data_ptr(), 20, 20, out.data_ptr(), work.data_ptr(), wb, st), [out], scratch)
+    e = eager[0].cpu().numpy()
+    assert complex(e[0], e[1]) == C.permanent(A)  # the host entry point runs the same plan
+
+
+def test_noalloc_device_in_a_graph():
+    rng = np.random.default_rng(4102)
+    A = random_complex(rng, 18, 18)
+    dA = torch.from_numpy(A).to(DEV)
+    out = torch.zeros(6, dtype=torch.float64, device=DEV)
+    scratch = torch.ones(1 << 16, dtype=torch.float64, device=DEV)
+    print("graph: pqb_perm_device_noalloc n=18", flush=True)
+    eager = _capture_and_replay(lambda st: C.permanent_device_noalloc(dA.data_ptr(), 18, 18, out.data_ptr(), st),
+                                [out], scratch)
+    e = eager[0].cpu().numpy()
+    assert complex(e[0], e[1]) == C.permanent(A)
+
+
+def test_batch_device_in_a_graph():
+    rng = np.random.default_rng(4103)
+    count, n = 64, 16
+    B = np.stack([random_complex(rng, n, n) for _ in range(count)])
+    dA = torch.from_numpy(B).to(DEV)
+    out = torch.zeros((count, 6), dtype=torch.float64, device=DEV)
+    wb = C.batch_device_workspace_bytes(count, n, n)
+    work = torch.zeros(wb, dtype=torch.uint8, device=DEV)
+    scratch = torch.ones(1 << 16, dtype=torch.float64, device=DEV)
+    print(f"graph: pqb_perm_batch_device {count} x n={n}", flush=True)
+    eager = _capture_and_replay(
+        lambda st: C.permanent_batch_device(dA.data_ptr(), count, n, n, out.data_ptr(), work.data_ptr(), wb, st),
+        [out], scratch)
+    e = eager[0].cpu().numpy()
+    for b in (0, 17, 63):
+        assert abs(complex(e[b, 0], e[b, 1]) - C.permanent(B[b])) <= 1e-12 * abs(C.permanent(B[b]))
+
+
+def test_grad_device_in_a_graph():
+    rng = np.random.default_rng(4104)
+    count, n = 4, 14
+    B = np.stack([random_complex(rng, n, n) for _ in range(count)])
+    dA = torch.from_numpy(B).to(DEV)
+    dp = torch.zeros(count, dtype=torch.complex128, device=DEV)
+    dG = torch.zeros_like(dA)
+    wb = C.grad_workspace_bytes(count, n)
+    work = torch.zeros(wb, dtype=torch.uint8, device=DEV)
+    scratch = torch.ones(1 << 16, dtype=torch.float64, device=DEV)
+    print(f"graph: pqb_perm_grad_device {count} x n={n}", flush=True)
+    eager = _capture_and_replay(
+        lambda st: C.permanent_grad_device(dA.data_ptr(), count, n, dp.data_ptr(), dG.data_ptr(), work.data_ptr(),
+                                           wb, st), [dp, dG], scratch)
+    P, G = C.permanent_grad_batch(B)
+    assert np.array_equal(eager[0].cpu().numpy(), P)
+    assert np.array_equal(eager[1].cpu().numpy(), G)
