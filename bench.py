@@ -11,7 +11,15 @@ Prints one JSON line with the throughput, the FP64 roofline fraction of the
 sweep kernel (algorithmic flops (8n-4) 2^(n-1) per permanent over the mean
 launch time measured with HIP events on the launch stream), and a CPU
 baseline: the restated reference TBB sweep (double, std::thread, one chunk per
-4 x threads as BBFGPermanentCalculator.hpp:120-134) timed on this host.
+4 x threads as BBFGPermanentCalculator.hpp:120-134) timed on this host.  The same
+line carries the other workloads' legs: the 30-photon repeated-row permanent, the
+4096-task batch against per-call launches, the gradient workload's forward and
+adjoint, the seeded sampler, the double-double and exact sweeps.
+
+Measurement modes that replace the benchmark: --fma-probe (FP64 issue rate at 1..8
+waves per SIMD), --walk-sweep (walk lengths of the 30-photon task), --power S
+(socket power and clock under each kernel for S seconds, amd-smi), --copy-mb M (a
+copy of known traffic for calibrating the HBM counters).
 """
 
 from __future__ import annotations
